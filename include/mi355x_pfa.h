@@ -1,0 +1,141 @@
+/*
+ * mi355x_pfa.h -- C ABI of the MI355X (gfx950) Polar Format focus library.
+ *
+ * The library replaces the focus stage of the reference SAR processor: the
+ * keystone resample (range pass, corner-turn, azimuth pass), the separable
+ * Hamming window, the two 1-D FFT passes with their corner-turn, and detection.
+ * Two arithmetic modes:
+ *   - fixed : bit-exact with mpfs/host/silicon_emulator.py (form_image and its
+ *             stage functions), integer arithmetic on int16 I/Q codes;
+ *   - float : src/form_image_pfa.py (pfa / resample_kspace / focus) in FP32,
+ *             with a float64 GPU geometry stage.
+ *
+ * Conventions
+ *   - Complex int16 data is interleaved I,Q (int16 I then int16 Q), the
+ *     reference's on-board sig.bin layout (serialize_inputs.py:181,
+ *     ddr_layout.py:346-364).  Complex float data is interleaved float32 re,im
+ *     (numpy complex64).  All arrays are C-contiguous, row-major.
+ *   - Functions without a _dev suffix take HOST pointers and are synchronous.
+ *     Functions with a _dev suffix take DEVICE pointers for the bulk arrays and
+ *     enqueue on the context's stream (see pfa_set_stream); small tables are
+ *     always host pointers and are copied by the library.
+ *   - Every function returns PFA_OK (0) or a negative pfa_status; the message
+ *     is available from pfa_last_error(ctx).  Nothing aborts or throws across
+ *     the ABI.  One context per host thread.
+ */
+#ifndef MI355X_PFA_H
+#define MI355X_PFA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PFA_ABI_VERSION 1
+
+typedef enum pfa_status {
+    PFA_OK = 0,
+    PFA_ERR_ARG = -1,        /* null pointer, bad flag, inconsistent sizes          */
+    PFA_ERR_DIMS = -2,       /* sizes outside what the kernels support             */
+    PFA_ERR_GEOMETRY = -3,   /* non-finite or degenerate geometry / tables         */
+    PFA_ERR_HIP = -4,        /* HIP runtime error (message carries hipGetErrorString) */
+    PFA_ERR_NOMEM = -5,      /* device allocation failed                           */
+    PFA_ERR_STATE = -6       /* wrong object kind for the call                     */
+} pfa_status;
+
+typedef struct pfa_ctx pfa_ctx;
+typedef struct pfa_plan pfa_plan;
+
+/* ------------------------------------------------------------------ context */
+int pfa_abi_version(void);
+/* Create a context on HIP device `device`.  It owns a stream and a grow-only
+ * device workspace that is reused across calls. */
+int pfa_create(pfa_ctx **ctx, int device);
+void pfa_destroy(pfa_ctx *ctx);
+const char *pfa_last_error(const pfa_ctx *ctx);
+/* Use an external hipStream_t (NULL restores the context's own stream). */
+int pfa_set_stream(pfa_ctx *ctx, void *hip_stream);
+int pfa_synchronize(pfa_ctx *ctx);
+/* Device time of the last kernel of `kind` enqueued by this context (ms,
+ * hipEvent pair on the context stream); kind: PFA_TIMER_*. */
+int pfa_timer_enable(pfa_ctx *ctx, int on);
+int pfa_timer_read(pfa_ctx *ctx, int kind, float *ms_total, int *launches);
+int pfa_timer_reset(pfa_ctx *ctx);
+enum { PFA_TIMER_RANGE = 0, PFA_TIMER_AZIMUTH = 1, PFA_TIMER_FFT_B = 2,
+       PFA_TIMER_DETECT = 3, PFA_TIMER_OTHER = 4, PFA_TIMER_KINDS = 5 };
+
+/* ------------------------------------------------------------------ tables */
+/* Truncated Q(bits-1) twiddles W_n^k, k < n/2, as fixedpoint.py:58-60 /
+ * :281-282 builds them: floor(cos(2*pi*k/n) * 2^(bits-1)) clipped to bits,
+ * floor(-sin(..) * 2^(bits-1)); out is n/2 interleaved (re, im) int32. */
+int pfa_twiddle_codes(int n, int bits, int32_t *out);
+
+/* ------------------------------------------------------------------ fixed mode
+ * Stage entry points, each a drop-in for the silicon_emulator.py function
+ * named.  Arrays of int16 codes are interleaved I,Q. */
+
+/* fixedpoint.fit_scale + floor + sat16 of a complex64 array
+ * (silicon_emulator.py:226-229): out_iq[count*2], *exp_out = block exponent. */
+int pfa_quantize_iq16(pfa_ctx *ctx, const float *sig_c64, int64_t count,
+                      int16_t *out_iq, int *exp_out);
+int pfa_quantize_iq16_dev(pfa_ctx *ctx, const float *sig_c64, int64_t count,
+                          int16_t *out_iq, int *exp_out);
+
+/* fft_pass_bfp (silicon_emulator.py:198-205 -> fixedpoint.py:266-289): per-row
+ * 16-bit BFP radix-2 FFT of `rows` rows of length n (power of two, 64..8192),
+ * then global renorm y >> (emax - e_row).  e_row[rows] receives the exponents. */
+int pfa_fft_pass_bfp(pfa_ctx *ctx, const int16_t *x_iq, int rows, int n,
+                     int16_t *y_iq, int32_t *e_row);
+int pfa_fft_pass_bfp_dev(pfa_ctx *ctx, const int16_t *x_iq, int rows, int n,
+                         int16_t *y_iq, int32_t *e_row);
+/* fft1d_bfp_hw_perrow alone (no global renorm). */
+int pfa_fft_bfp_perrow(pfa_ctx *ctx, const int16_t *x_iq, int rows, int n,
+                       int16_t *y_iq, int32_t *e_row);
+
+/* window_fixed (silicon_emulator.py:165-195) on g2 [rows][cols] (range rows,
+ * cross columns); hamr[rows], hamc[cols] are the floor(hamming*32768) tapers
+ * (int32: the centre tap may be 32768). */
+int pfa_window_fixed(pfa_ctx *ctx, const int16_t *g2_iq, int rows, int cols,
+                     const int32_t *hamr, const int32_t *hamc, int16_t *out_iq);
+
+/* detect_fixed (silicon_emulator.py:208-213): floor(sqrt(I^2+Q^2)) -> uint16. */
+int pfa_detect_fixed(pfa_ctx *ctx, const int16_t *z_iq, int64_t count, uint16_t *out);
+
+/* Scene tables of silicon_emulator.py:128-140 (the board's DDR roles,
+ * serialize_inputs.py:203-230): f0, df, pr, tan_s float32[m]; inv_order
+ * int32[m]; krp, kcp float32[grid]; hamr, hamc int32[grid]; sinc_tab int16
+ * [256][32] (gen_resample_vectors.sinc_table_q15) or NULL when neither pass
+ * uses the sinc gather. */
+#define PFA_FIXED_RANGE_SINC  0x1   /* pass 1: 32-tap sinc (else 2-tap lerp)       */
+#define PFA_FIXED_AZ_SINC     0x2   /* pass 2: 32-tap sinc (else 2-tap lerp)       */
+#define PFA_FIXED_LERP_EXACT  0x4   /* pass-1 lerp product exact (the stage fed a
+                                       complex128 line); default is form_image's
+                                       float32 product of a complex64 line        */
+int pfa_plan_fixed(pfa_ctx *ctx, pfa_plan **plan, int m, int n, int grid,
+                   const float *f0, const float *df, const float *pr,
+                   const float *tan_s, const int32_t *inv_order,
+                   const float *krp, const float *kcp,
+                   const int32_t *hamr, const int32_t *hamc,
+                   const int16_t *sinc_tab, int flags);
+
+/* resample_fixed (silicon_emulator.py:125-156): sig_iq[m][n] -> g2_iq
+ * [grid range][grid cross]. */
+int pfa_resample_fixed(pfa_plan *plan, const int16_t *sig_iq, int16_t *g2_iq);
+
+/* form_image after quantisation (silicon_emulator.py:233-249): resample ->
+ * window -> FFT along cross (per range row) -> corner-turn -> FFT along range
+ * -> detect.  out[grid range][grid cross] uint16; exp_a[grid], exp_b[grid]
+ * receive the per-row exponents of the two FFT passes (may be NULL). */
+int pfa_form_image_fixed(pfa_plan *plan, const int16_t *sig_iq, uint16_t *out,
+                         int32_t *exp_a, int32_t *exp_b);
+int pfa_form_image_fixed_dev(pfa_plan *plan, const int16_t *sig_iq, uint16_t *out,
+                             int32_t *exp_a, int32_t *exp_b);
+
+void pfa_plan_destroy(pfa_plan *plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MI355X_PFA_H */

@@ -1,0 +1,477 @@
+// pfa_api.cpp -- host side of the C ABI (include/mi355x_pfa.h): contexts,
+// workspace, plans, argument validation and the host-pointer wrappers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "pfa_internal.hpp"
+
+// ------------------------------------------------------------------ errors / workspace
+int pfa_fail(pfa_ctx *ctx, int code, const std::string &msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+int pfa_hip_fail(pfa_ctx *ctx, hipError_t e, const char *what) {
+    return pfa_fail(ctx, e == hipErrorOutOfMemory ? PFA_ERR_NOMEM : PFA_ERR_HIP,
+                    std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void *pfa_ws(pfa_ctx *ctx, const char *slot, size_t bytes, int *status) {
+    DevBuf &b = ctx->ws[slot];
+    if (b.bytes >= bytes && b.ptr) return b.ptr;
+    if (b.ptr) {
+        hipStreamSynchronize(ctx->stream);
+        hipFree(b.ptr);
+        b.ptr = nullptr;
+        b.bytes = 0;
+    }
+    size_t want = std::max<size_t>(bytes, 256);
+    hipError_t e = hipMalloc(&b.ptr, want);
+    if (e != hipSuccess) {
+        b.ptr = nullptr;
+        *status = pfa_hip_fail(ctx, e, (std::string("workspace ") + slot).c_str());
+        return nullptr;
+    }
+    b.bytes = want;
+    return b.ptr;
+}
+
+static hipEvent_t take_event(pfa_ctx *ctx) {
+    if (!ctx->event_pool.empty()) {
+        hipEvent_t e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    hipEventCreate(&e);
+    return e;
+}
+
+void pfa_timer_begin(pfa_ctx *ctx, int kind) {
+    if (!ctx->timing) return;
+    TimerMark mk{kind, take_event(ctx), take_event(ctx)};
+    hipEventRecord(mk.start, ctx->stream);
+    ctx->marks.push_back(mk);
+}
+
+void pfa_timer_end(pfa_ctx *ctx, int kind) {
+    if (!ctx->timing || ctx->marks.empty()) return;
+    TimerMark &mk = ctx->marks.back();
+    if (mk.kind == kind) hipEventRecord(mk.stop, ctx->stream);
+}
+
+// ------------------------------------------------------------------ context
+extern "C" int pfa_abi_version(void) { return PFA_ABI_VERSION; }
+
+extern "C" int pfa_create(pfa_ctx **out, int device) {
+    if (!out) return PFA_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PFA_ERR_HIP;
+    if (device < 0 || device >= ndev) return PFA_ERR_ARG;
+    pfa_ctx *ctx = new (std::nothrow) pfa_ctx();
+    if (!ctx) return PFA_ERR_NOMEM;
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->own_stream) != hipSuccess) {
+        delete ctx;
+        return PFA_ERR_HIP;
+    }
+    ctx->stream = ctx->own_stream;
+    if (hipMalloc(&ctx->d_counters, 64 * sizeof(int)) != hipSuccess ||
+        hipMalloc(&ctx->d_maxbits, 64 * sizeof(unsigned)) != hipSuccess) {
+        hipStreamDestroy(ctx->own_stream);
+        delete ctx;
+        return PFA_ERR_NOMEM;
+    }
+    *out = ctx;
+    return PFA_OK;
+}
+
+extern "C" void pfa_destroy(pfa_ctx *ctx) {
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    for (auto &kv : ctx->ws)
+        if (kv.second.ptr) hipFree(kv.second.ptr);
+    for (auto &mk : ctx->marks) {
+        hipEventDestroy(mk.start);
+        hipEventDestroy(mk.stop);
+    }
+    for (auto e : ctx->event_pool) hipEventDestroy(e);
+    hipFree(ctx->d_counters);
+    hipFree(ctx->d_maxbits);
+    hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+}
+
+extern "C" const char *pfa_last_error(const pfa_ctx *ctx) {
+    return ctx ? ctx->err.c_str() : "null context";
+}
+
+extern "C" int pfa_set_stream(pfa_ctx *ctx, void *s) {
+    if (!ctx) return PFA_ERR_ARG;
+    ctx->stream = s ? (hipStream_t)s : ctx->own_stream;
+    return PFA_OK;
+}
+
+extern "C" int pfa_synchronize(pfa_ctx *ctx) {
+    if (!ctx) return PFA_ERR_ARG;
+    PFA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PFA_OK;
+}
+
+extern "C" int pfa_timer_enable(pfa_ctx *ctx, int on) {
+    if (!ctx) return PFA_ERR_ARG;
+    ctx->timing = on != 0;
+    return PFA_OK;
+}
+
+extern "C" int pfa_timer_reset(pfa_ctx *ctx) {
+    if (!ctx) return PFA_ERR_ARG;
+    PFA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (auto &mk : ctx->marks) {
+        ctx->event_pool.push_back(mk.start);
+        ctx->event_pool.push_back(mk.stop);
+    }
+    ctx->marks.clear();
+    return PFA_OK;
+}
+
+extern "C" int pfa_timer_read(pfa_ctx *ctx, int kind, float *ms_total, int *launches) {
+    if (!ctx || !ms_total || !launches || kind < 0 || kind >= PFA_TIMER_KINDS) return PFA_ERR_ARG;
+    PFA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    float tot = 0.f;
+    int cnt = 0;
+    for (auto &mk : ctx->marks) {
+        if (mk.kind != kind) continue;
+        float ms = 0.f;
+        PFA_HIP(ctx, hipEventElapsedTime(&ms, mk.start, mk.stop));
+        tot += ms;
+        ++cnt;
+    }
+    *ms_total = tot;
+    *launches = cnt;
+    return PFA_OK;
+}
+
+// ------------------------------------------------------------------ tables
+extern "C" int pfa_twiddle_codes(int n, int bits, int32_t *out) {
+    if (n < 2 || (n & (n - 1)) || bits < 2 || bits > 30 || !out) return PFA_ERR_ARG;
+    const double scale = std::ldexp(1.0, bits - 1);
+    const double full = scale - 1.0;
+    for (int k = 0; k < n / 2; ++k) {
+        // the angle exactly as numpy forms -2j*pi*k/n: fl(2*pi*k) / n
+        const double th = (2.0 * M_PI) * (double)k / (double)n;
+        const double re = std::floor(std::cos(th) * scale);
+        const double im = std::floor(-std::sin(th) * scale);
+        out[2 * k] = (int32_t)std::min(std::max(re, -full - 1.0), full);
+        out[2 * k + 1] = (int32_t)std::min(std::max(im, -full - 1.0), full);
+    }
+    return PFA_OK;
+}
+
+// ------------------------------------------------------------------ helpers
+static bool pow2_in(int n, int lo, int hi) { return n >= lo && n <= hi && (n & (n - 1)) == 0; }
+
+template <typename T>
+static T *upload(pfa_plan *p, const T *h, size_t count, int *st) {
+    void *d = nullptr;
+    hipError_t e = hipMalloc(&d, std::max<size_t>(count * sizeof(T), 64));
+    if (e != hipSuccess) {
+        *st = pfa_hip_fail(p->ctx, e, "plan table");
+        return nullptr;
+    }
+    p->owned.push_back(d);
+    if (count) {
+        e = hipMemcpy(d, h, count * sizeof(T), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            *st = pfa_hip_fail(p->ctx, e, "plan table upload");
+            return nullptr;
+        }
+    }
+    return (T *)d;
+}
+
+#define CK(x)                     \
+    do {                          \
+        int s_ = (x);             \
+        if (s_ != PFA_OK) return s_; \
+    } while (0)
+
+// ------------------------------------------------------------------ fixed: stage functions
+extern "C" int pfa_quantize_iq16_dev(pfa_ctx *ctx, const float *sig, int64_t count, int16_t *out,
+                                     int *exp_out) {
+    if (!ctx || (count && (!sig || !out)) || !exp_out || count < 0) return pfa_fail(ctx, PFA_ERR_ARG, "quantize: bad argument");
+    return launch_quantize(ctx, sig, count * 2, out, exp_out);
+}
+
+extern "C" int pfa_quantize_iq16(pfa_ctx *ctx, const float *sig, int64_t count, int16_t *out,
+                                 int *exp_out) {
+    if (!ctx || (count && (!sig || !out)) || !exp_out || count < 0) return pfa_fail(ctx, PFA_ERR_ARG, "quantize: bad argument");
+    int st = PFA_OK;
+    const size_t in_b = (size_t)count * 8, out_b = (size_t)count * 4;
+    float *d_in = (float *)pfa_ws(ctx, "io_in", in_b, &st);
+    if (!d_in) return st;
+    int16_t *d_out = (int16_t *)pfa_ws(ctx, "io_out", out_b, &st);
+    if (!d_out) return st;
+    PFA_HIP(ctx, hipMemcpyAsync(d_in, sig, in_b, hipMemcpyHostToDevice, ctx->stream));
+    CK(launch_quantize(ctx, d_in, count * 2, d_out, exp_out));
+    PFA_HIP(ctx, hipMemcpyAsync(out, d_out, out_b, hipMemcpyDeviceToHost, ctx->stream));
+    PFA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PFA_OK;
+}
+
+static int fft_rows_dev(pfa_ctx *ctx, const int16_t *x, int rows, int n, int16_t *y, int32_t *e,
+                        bool renorm) {
+    if (!ctx) return PFA_ERR_ARG;
+    if (!pow2_in(n, 64, 8192)) return pfa_fail(ctx, PFA_ERR_DIMS, "FFT length must be a power of two in 64..8192");
+    if (rows < 0 || (rows && (!x || !y || !e))) return pfa_fail(ctx, PFA_ERR_ARG, "fft: bad argument");
+    if (rows == 0) return PFA_OK;
+    int *emax = ctx->d_counters + 8;
+    CK(launch_zero_i32(ctx, emax, 1));
+    FixedFFTArgs a{x, n, y, e, emax, rows, n};
+    CK(launch_fft_bfp_rows(ctx, a));
+    if (renorm) CK(launch_renorm_rows(ctx, y, e, emax, rows, n));
+    return PFA_OK;
+}
+
+extern "C" int pfa_fft_pass_bfp_dev(pfa_ctx *ctx, const int16_t *x, int rows, int n, int16_t *y,
+                                    int32_t *e) {
+    return fft_rows_dev(ctx, x, rows, n, y, e, true);
+}
+
+static int fft_rows_host(pfa_ctx *ctx, const int16_t *x, int rows, int n, int16_t *y, int32_t *e,
+                         bool renorm) {
+    if (!ctx) return PFA_ERR_ARG;
+    if (rows < 0 || (rows && (!x || !y || !e))) return pfa_fail(ctx, PFA_ERR_ARG, "fft: bad argument");
+    if (!pow2_in(n, 64, 8192)) return pfa_fail(ctx, PFA_ERR_DIMS, "FFT length must be a power of two in 64..8192");
+    if (rows == 0) return PFA_OK;
+    int st = PFA_OK;
+    const size_t b = (size_t)rows * n * 4;
+    void *dx = pfa_ws(ctx, "io_in", b, &st);
+    if (!dx) return st;
+    void *dy = pfa_ws(ctx, "io_out", b, &st);
+    if (!dy) return st;
+    int32_t *de = (int32_t *)pfa_ws(ctx, "io_e", (size_t)rows * 4, &st);
+    if (!de) return st;
+    PFA_HIP(ctx, hipMemcpyAsync(dx, x, b, hipMemcpyHostToDevice, ctx->stream));
+    CK(fft_rows_dev(ctx, (const int16_t *)dx, rows, n, (int16_t *)dy, de, renorm));
+    PFA_HIP(ctx, hipMemcpyAsync(y, dy, b, hipMemcpyDeviceToHost, ctx->stream));
+    PFA_HIP(ctx, hipMemcpyAsync(e, de, (size_t)rows * 4, hipMemcpyDeviceToHost, ctx->stream));
+    PFA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PFA_OK;
+}
+
+extern "C" int pfa_fft_pass_bfp(pfa_ctx *ctx, const int16_t *x, int rows, int n, int16_t *y,
+                                int32_t *e) {
+    return fft_rows_host(ctx, x, rows, n, y, e, true);
+}
+
+extern "C" int pfa_fft_bfp_perrow(pfa_ctx *ctx, const int16_t *x, int rows, int n, int16_t *y,
+                                  int32_t *e) {
+    return fft_rows_host(ctx, x, rows, n, y, e, false);
+}
+
+extern "C" int pfa_window_fixed(pfa_ctx *ctx, const int16_t *g2, int rows, int cols,
+                                const int32_t *hamr, const int32_t *hamc, int16_t *out) {
+    if (!ctx) return PFA_ERR_ARG;
+    if (rows < 0 || cols < 0 || ((rows && cols) && (!g2 || !hamr || !hamc || !out)))
+        return pfa_fail(ctx, PFA_ERR_ARG, "window: bad argument");
+    for (int i = 0; i < rows; ++i)
+        if (hamr[i] < 0 || hamr[i] > 32768) return pfa_fail(ctx, PFA_ERR_GEOMETRY, "window: hamr outside [0, 32768]");
+    for (int i = 0; i < cols; ++i)
+        if (hamc[i] < 0 || hamc[i] > 32768) return pfa_fail(ctx, PFA_ERR_GEOMETRY, "window: hamc outside [0, 32768]");
+    const size_t b = (size_t)rows * cols * 4;
+    if (!b) return PFA_OK;
+    int st = PFA_OK;
+    void *dx = pfa_ws(ctx, "io_in", b, &st);
+    if (!dx) return st;
+    void *dy = pfa_ws(ctx, "io_out", b, &st);
+    if (!dy) return st;
+    int32_t *dh = (int32_t *)pfa_ws(ctx, "io_ham", (size_t)(rows + cols) * 4, &st);
+    if (!dh) return st;
+    PFA_HIP(ctx, hipMemcpyAsync(dx, g2, b, hipMemcpyHostToDevice, ctx->stream));
+    PFA_HIP(ctx, hipMemcpyAsync(dh, hamr, (size_t)rows * 4, hipMemcpyHostToDevice, ctx->stream));
+    PFA_HIP(ctx, hipMemcpyAsync(dh + rows, hamc, (size_t)cols * 4, hipMemcpyHostToDevice, ctx->stream));
+    CK(launch_window_fixed(ctx, dx, rows, cols, dh, dh + rows, dy));
+    PFA_HIP(ctx, hipMemcpyAsync(out, dy, b, hipMemcpyDeviceToHost, ctx->stream));
+    PFA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PFA_OK;
+}
+
+extern "C" int pfa_detect_fixed(pfa_ctx *ctx, const int16_t *z, int64_t count, uint16_t *out) {
+    if (!ctx) return PFA_ERR_ARG;
+    if (count < 0 || (count && (!z || !out))) return pfa_fail(ctx, PFA_ERR_ARG, "detect: bad argument");
+    if (!count) return PFA_OK;
+    int st = PFA_OK;
+    void *dz = pfa_ws(ctx, "io_in", (size_t)count * 4, &st);
+    if (!dz) return st;
+    void *dy = pfa_ws(ctx, "io_out", (size_t)count * 2, &st);
+    if (!dy) return st;
+    PFA_HIP(ctx, hipMemcpyAsync(dz, z, (size_t)count * 4, hipMemcpyHostToDevice, ctx->stream));
+    CK(launch_detect_fixed(ctx, dz, count, (uint16_t *)dy));
+    PFA_HIP(ctx, hipMemcpyAsync(out, dy, (size_t)count * 2, hipMemcpyDeviceToHost, ctx->stream));
+    PFA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PFA_OK;
+}
+
+// ------------------------------------------------------------------ fixed: plan + pipeline
+extern "C" int pfa_plan_fixed(pfa_ctx *ctx, pfa_plan **out, int m, int n, int grid,
+                              const float *f0, const float *df, const float *pr,
+                              const float *tan_s, const int32_t *inv_order, const float *krp,
+                              const float *kcp, const int32_t *hamr, const int32_t *hamc,
+                              const int16_t *sinc_tab, int flags) {
+    if (!ctx || !out) return PFA_ERR_ARG;
+    *out = nullptr;
+    if (!f0 || !df || !pr || !tan_s || !inv_order || !krp || !kcp || !hamr || !hamc)
+        return pfa_fail(ctx, PFA_ERR_ARG, "plan_fixed: null table");
+    if (flags & ~(PFA_FIXED_RANGE_SINC | PFA_FIXED_AZ_SINC | PFA_FIXED_LERP_EXACT))
+        return pfa_fail(ctx, PFA_ERR_ARG, "plan_fixed: unknown flag");
+    if (!pow2_in(grid, 64, 8192)) return pfa_fail(ctx, PFA_ERR_DIMS, "grid must be a power of two in 64..8192");
+    if (m < 2 || n < 2 || m > grid || n > grid)
+        return pfa_fail(ctx, PFA_ERR_DIMS, "plan_fixed: need 2 <= m, n <= grid");
+    if ((flags & (PFA_FIXED_RANGE_SINC | PFA_FIXED_AZ_SINC)) && !sinc_tab)
+        return pfa_fail(ctx, PFA_ERR_ARG, "plan_fixed: sinc flags need the 256x32 tap table");
+    if (sinc_tab) {
+        for (int ph = 0; ph < 256; ++ph) {
+            int sum = 0;
+            for (int t = 0; t < 32; ++t) sum += sinc_tab[ph * 32 + t];
+            if (sum != 32768) return pfa_fail(ctx, PFA_ERR_GEOMETRY, "sinc table phase does not sum to 2^15");
+        }
+    }
+    for (int i = 0; i < m; ++i)
+        if (!std::isfinite(f0[i]) || !std::isfinite(df[i]) || !std::isfinite(pr[i]) || !std::isfinite(tan_s[i]))
+            return pfa_fail(ctx, PFA_ERR_GEOMETRY, "plan_fixed: non-finite per-pulse table");
+    for (int i = 0; i < grid; ++i)
+        if (!std::isfinite(krp[i]) || !std::isfinite(kcp[i]))
+            return pfa_fail(ctx, PFA_ERR_GEOMETRY, "plan_fixed: non-finite KR/KC grid");
+    for (int i = 0; i < grid; ++i)
+        if (hamr[i] < 0 || hamr[i] > 32768 || hamc[i] < 0 || hamc[i] > 32768)
+            return pfa_fail(ctx, PFA_ERR_GEOMETRY, "plan_fixed: taper outside [0, 32768]");
+    std::vector<int32_t> order(m, -1);
+    for (int i = 0; i < m; ++i) {
+        const int p = inv_order[i];
+        if (p < 0 || p >= m || order[p] != -1) return pfa_fail(ctx, PFA_ERR_GEOMETRY, "inv_order is not a permutation");
+        order[p] = i;
+    }
+    // a = f32(f32(2*pr)/f32(C)) (silicon_emulator.py:144, NumPy 2 float32 scalar math)
+    std::vector<float> a32(m);
+    const volatile float c32 = 299792458.0f;
+    for (int i = 0; i < m; ++i) {
+        volatile float two = 2.0f * pr[i];
+        a32[i] = two / c32;
+        if (!std::isfinite(a32[i]) || a32[i] == 0.0f || df[i] == 0.0f)
+            return pfa_fail(ctx, PFA_ERR_GEOMETRY, "plan_fixed: degenerate range geometry (pr or df zero)");
+    }
+    pfa_plan *p = new (std::nothrow) pfa_plan();
+    if (!p) return PFA_ERR_NOMEM;
+    p->kind = PLAN_FIXED;
+    p->ctx = ctx;
+    p->m = m; p->n = n; p->grid = grid; p->flags = flags;
+    int st = PFA_OK;
+    bool ok = (p->a32 = upload(p, a32.data(), m, &st)) && (p->f0 = upload(p, f0, m, &st)) &&
+              (p->df = upload(p, df, m, &st)) && (p->tan_s = upload(p, tan_s, m, &st)) &&
+              (p->order = upload(p, order.data(), m, &st)) && (p->krp = upload(p, krp, grid, &st)) &&
+              (p->kcp = upload(p, kcp, grid, &st)) && (p->hamr = upload(p, hamr, grid, &st)) &&
+              (p->hamc = upload(p, hamc, grid, &st));
+    if (ok && sinc_tab) ok = (p->sinc = upload(p, sinc_tab, 256 * 32, &st)) != nullptr;
+    if (!ok) {
+        pfa_plan_destroy(p);
+        return st;
+    }
+    *out = p;
+    return PFA_OK;
+}
+
+extern "C" void pfa_plan_destroy(pfa_plan *p) {
+    if (!p) return;
+    hipStreamSynchronize(p->ctx->stream);
+    for (void *d : p->owned) hipFree(d);
+    delete p;
+}
+
+static int ldp_of(int m) { return (m + 63) / 64 * 64; }
+
+// resample only (stage drop-in); g2 device [grid][grid]
+static int resample_fixed_dev(pfa_plan *p, const void *sig, void *g2) {
+    pfa_ctx *ctx = p->ctx;
+    int st = PFA_OK;
+    const int G = p->grid, ldp = ldp_of(p->m);
+    void *g1t = pfa_ws(ctx, "g1t", (size_t)G * G * 4, &st);
+    if (!g1t) return st;
+    CK(launch_fixed_range(p, sig, g1t, ldp));
+    CK(launch_fixed_azimuth_fft(p, g1t, ldp, g2, nullptr, nullptr, false, false));
+    return PFA_OK;
+}
+
+extern "C" int pfa_resample_fixed(pfa_plan *p, const int16_t *sig, int16_t *g2) {
+    if (!p || p->kind != PLAN_FIXED) return PFA_ERR_STATE;
+    pfa_ctx *ctx = p->ctx;
+    if (!sig || !g2) return pfa_fail(ctx, PFA_ERR_ARG, "resample: null pointer");
+    int st = PFA_OK;
+    const size_t in_b = (size_t)p->m * p->n * 4, out_b = (size_t)p->grid * p->grid * 4;
+    void *ds = pfa_ws(ctx, "io_in", in_b, &st);
+    if (!ds) return st;
+    void *dg = pfa_ws(ctx, "io_out", out_b, &st);
+    if (!dg) return st;
+    PFA_HIP(ctx, hipMemcpyAsync(ds, sig, in_b, hipMemcpyHostToDevice, ctx->stream));
+    CK(resample_fixed_dev(p, ds, dg));
+    PFA_HIP(ctx, hipMemcpyAsync(g2, dg, out_b, hipMemcpyDeviceToHost, ctx->stream));
+    PFA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PFA_OK;
+}
+
+extern "C" int pfa_form_image_fixed_dev(pfa_plan *p, const int16_t *sig, uint16_t *out,
+                                        int32_t *exp_a, int32_t *exp_b) {
+    if (!p || p->kind != PLAN_FIXED) return PFA_ERR_STATE;
+    pfa_ctx *ctx = p->ctx;
+    if (!sig || !out) return pfa_fail(ctx, PFA_ERR_ARG, "form_image: null pointer");
+    int st = PFA_OK;
+    const int G = p->grid, ldp = ldp_of(p->m);
+    void *g1t = pfa_ws(ctx, "g1t", (size_t)G * G * 4, &st);
+    if (!g1t) return st;
+    void *ya = pfa_ws(ctx, "ya", (size_t)G * G * 4, &st);
+    if (!ya) return st;
+    int32_t *ea = exp_a ? exp_a : (int32_t *)pfa_ws(ctx, "ea", (size_t)G * 4, &st);
+    if (!ea) return st;
+    int32_t *eb = exp_b ? exp_b : (int32_t *)pfa_ws(ctx, "eb", (size_t)G * 4, &st);
+    if (!eb) return st;
+    int *emax = ctx->d_counters;          // [0] pass A, [1] pass B
+    CK(launch_zero_i32(ctx, emax, 2));
+    CK(launch_fixed_range(p, sig, g1t, ldp));
+    CK(launch_fixed_azimuth_fft(p, g1t, ldp, ya, ea, emax, true, true));
+    void *yb = g1t;                       // the pass-1 buffer is dead after pass 2
+    CK(launch_fixed_fft_b(p, ya, ea, emax, yb, eb, emax + 1));
+    CK(launch_fixed_detect_t(p, yb, eb, emax + 1, out));
+    return PFA_OK;
+}
+
+extern "C" int pfa_form_image_fixed(pfa_plan *p, const int16_t *sig, uint16_t *out,
+                                    int32_t *exp_a, int32_t *exp_b) {
+    if (!p || p->kind != PLAN_FIXED) return PFA_ERR_STATE;
+    pfa_ctx *ctx = p->ctx;
+    if (!sig || !out) return pfa_fail(ctx, PFA_ERR_ARG, "form_image: null pointer");
+    int st = PFA_OK;
+    const int G = p->grid;
+    const size_t in_b = (size_t)p->m * p->n * 4, out_b = (size_t)G * G * 2;
+    void *ds = pfa_ws(ctx, "io_in", in_b, &st);
+    if (!ds) return st;
+    void *dout = pfa_ws(ctx, "io_out", out_b, &st);
+    if (!dout) return st;
+    int32_t *de = (int32_t *)pfa_ws(ctx, "io_e", (size_t)G * 8, &st);
+    if (!de) return st;
+    PFA_HIP(ctx, hipMemcpyAsync(ds, sig, in_b, hipMemcpyHostToDevice, ctx->stream));
+    CK(pfa_form_image_fixed_dev(p, (const int16_t *)ds, (uint16_t *)dout, de, de + G));
+    PFA_HIP(ctx, hipMemcpyAsync(out, dout, out_b, hipMemcpyDeviceToHost, ctx->stream));
+    if (exp_a) PFA_HIP(ctx, hipMemcpyAsync(exp_a, de, (size_t)G * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (exp_b) PFA_HIP(ctx, hipMemcpyAsync(exp_b, de + G, (size_t)G * 4, hipMemcpyDeviceToHost, ctx->stream));
+    PFA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PFA_OK;
+}

@@ -1,0 +1,685 @@
+// pfa_fixed.hip -- gfx950 kernels of the bit-exact fixed-point focus path.
+//
+// Arithmetic contract: mpfs/host/silicon_emulator.py (form_image :216-253 and
+// its stage functions), src/fixedpoint.py:266-289, serialize_inputs.py:67-94,
+// gen_resample_vectors.py:257-296.  Complex int16 data is short2 (x = I, y = Q).
+//
+// Kernels
+//   k_range_resample    pass 1 (F3+F4+F5): FP64 positions, bracket, 2-tap lerp or
+//                       32-tap sinc, pulse-sorted rows written corner-turned
+//                       through an LDS tile: g1t[range q][sorted pulse p].
+//   k_azimuth_fft       pass 2 (F6+F7) + window (F8) + BFP FFT along cross (F9a),
+//                       one range row per workgroup, everything LDS-resident.
+//   k_fft_b             BFP FFT along range (F9b) for one cross column per
+//                       workgroup, pass-A renorm folded into its column load.
+//   k_detect_t          pass-B renorm + detect (F11) + LDS-tiled transpose back
+//                       to [range][cross].
+//   k_fft_rows          stand-alone per-row BFP FFT (fft_pass_bfp / hw_perrow).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "pfa_internal.hpp"
+
+namespace {
+
+constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v >> 1); }
+
+__device__ __forceinline__ int sat16(int v) { return min(max(v, -32768), 32767); }
+__device__ __forceinline__ int wrap16(int v) { return (int)(short)(v & 0xffff); }
+__device__ __forceinline__ int pad32(int i) { return i + (i >> 5); }
+
+// ------------------------------------------------------------------ BFP FFT core
+// fixedpoint.fft1d_bfp_hw_perrow for one line of N = 2^L, 16-bit data and
+// 16-bit truncated twiddles.  NT threads hold E = N/NT elements each; stages
+// whose butterfly partner bit is among a thread's E local index bits run in
+// registers, and the local bit window moves through LDS between them.  Every
+// stage ends with the line-wide max that sets the shift (0/1/2) the reference
+// applies before the next stage.
+template <int L>
+struct Fft {
+    static constexpr int N = 1 << L;
+    static constexpr int NT = (N / 2 < 256) ? N / 2 : 256;
+    static constexpr int E = N / NT;
+    static constexpr int P = ilog2c(E);
+    static constexpr int NB = NT < 64 ? 64 : NT;
+    static constexpr int NW = NB / 64;
+    static constexpr int DATA = N + N / 32;          // padded short2 slots
+
+    static __device__ __forceinline__ int idx(int t, int l, int b) {
+        const int lo = t & ((1 << b) - 1);
+        const int hi = t >> b;
+        return lo | (l << b) | (hi << (b + P));
+    }
+    static __device__ __forceinline__ int base_of(int s) {     // layout of stage s
+        const int b = ((s - 1) / P) * P;
+        return b < L - P ? b : L - P;
+    }
+
+    // data: padded natural-order input (read), reused as exchange buffer.
+    // On return xr/xi hold the output at idx(t, l, L - P); e = summed shifts.
+    static __device__ void run(short2 *data, const short2 *tw, int *red, int (&xr)[E],
+                               int (&xi)[E], int &e) {
+        const int t = threadIdx.x;
+        const bool act = t < NT;
+        if (act) {
+#pragma unroll
+            for (int l = 0; l < E; ++l) {
+                const int i = idx(t, l, 0);
+                const int src = (int)(__brev((unsigned)i) >> (32 - L));
+                const short2 v = data[pad32(src)];
+                xr[l] = v.x;
+                xi[l] = v.y;
+            }
+        }
+        e = 0;
+        int b = 0;
+#pragma unroll
+        for (int s = 1; s <= L; ++s) {
+            const int nb = base_of(s);
+            if (nb != b) {                      // move the local bit window
+                if (act) {
+#pragma unroll
+                    for (int l = 0; l < E; ++l)
+                        data[pad32(idx(t, l, b))] = make_short2((short)xr[l], (short)xi[l]);
+                }
+                __syncthreads();
+                if (act) {
+#pragma unroll
+                    for (int l = 0; l < E; ++l) {
+                        const short2 v = data[pad32(idx(t, l, nb))];
+                        xr[l] = v.x;
+                        xi[l] = v.y;
+                    }
+                }
+                b = nb;
+            }
+            const int q = (s - 1) - b;
+            const int h = 1 << (s - 1);
+            int lmax = 0;
+            if (act) {
+#pragma unroll
+                for (int l = 0; l < E; ++l) {
+                    if (l & (1 << q)) continue;
+                    const int l1 = l | (1 << q);
+                    const int j = idx(t, l, b) & (h - 1);
+                    const short2 w = tw[j << (L - s)];
+                    const int br = xr[l1], bi = xi[l1];
+                    const int tr = sat16((w.x * br - w.y * bi) >> 15);
+                    const int ti = sat16((w.x * bi + w.y * br) >> 15);
+                    const int ar = xr[l], ai = xi[l];
+                    xr[l] = ar + tr;
+                    xi[l] = ai + ti;
+                    xr[l1] = ar - tr;
+                    xi[l1] = ai - ti;
+                    lmax = max(lmax, max(max(abs(ar + tr), abs(ai + ti)),
+                                         max(abs(ar - tr), abs(ai - ti))));
+                }
+            }
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) lmax = max(lmax, __shfl_xor(lmax, off, 64));
+            if ((t & 63) == 0) red[(s & 1) * NW + (t >> 6)] = lmax;
+            __syncthreads();
+            int mx = red[(s & 1) * NW];
+#pragma unroll
+            for (int w = 1; w < NW; ++w) mx = max(mx, red[(s & 1) * NW + w]);
+            const int sh = mx <= 32766 ? 0 : (mx <= 65533 ? 1 : 2);
+            if (sh && act) {
+#pragma unroll
+                for (int l = 0; l < E; ++l) {
+                    xr[l] = sat16(xr[l] >> sh);
+                    xi[l] = sat16(xi[l] >> sh);
+                }
+            }
+            e += sh;
+        }
+    }
+};
+
+template <int L>
+__device__ __forceinline__ void load_twiddles(short2 *tw, const short2 *twg) {
+    constexpr int H = (1 << L) / 2;
+    for (int i = threadIdx.x; i < H; i += blockDim.x) tw[i] = twg[i];
+}
+
+// ------------------------------------------------------------------ stand-alone row FFT
+template <int L>
+__global__ __launch_bounds__(Fft<L>::NB) void k_fft_rows(const short2 *__restrict__ in, int ld,
+                                                        short2 *__restrict__ out,
+                                                        int32_t *__restrict__ e_row,
+                                                        int *__restrict__ emax,
+                                                        const short2 *__restrict__ twg) {
+    using F = Fft<L>;
+    __shared__ short2 data[F::DATA];
+    __shared__ short2 tw[F::N / 2];
+    __shared__ int red[2 * F::NW];
+    const int row = blockIdx.x;
+    for (int i = threadIdx.x; i < F::N; i += blockDim.x) data[pad32(i)] = in[(size_t)row * ld + i];
+    load_twiddles<L>(tw, twg);
+    __syncthreads();
+    int xr[F::E], xi[F::E], e;
+    F::run(data, tw, red, xr, xi, e);
+    if (threadIdx.x < F::NT) {
+#pragma unroll
+        for (int l = 0; l < F::E; ++l)
+            out[(size_t)row * F::N + F::idx(threadIdx.x, l, L - F::P)] =
+                make_short2((short)xr[l], (short)xi[l]);
+    }
+    if (threadIdx.x == 0) {
+        e_row[row] = e;
+        if (emax) atomicMax(emax, e);
+    }
+}
+
+__global__ void k_renorm_rows(short2 *y, const int32_t *e_row, const int *emax, int n,
+                              int64_t total) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int sh = *emax - e_row[i / n];
+    short2 v = y[i];
+    v.x = (short)sat16(v.x >> sh);
+    v.y = (short)sat16(v.y >> sh);
+    y[i] = v;
+}
+
+// ------------------------------------------------------------------ elementwise stages
+__global__ void k_window(const short2 *g2, int rows, int cols, const int32_t *hamr,
+                         const int32_t *hamc, short2 *out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)rows * cols) return;
+    const int j = (int)(i / cols), k = (int)(i % cols);
+    const int cw = wrap16((hamr[j] * hamc[k]) >> 15);
+    const short2 v = g2[i];
+    out[i] = make_short2((short)wrap16((v.x * cw) >> 15), (short)wrap16((v.y * cw) >> 15));
+}
+
+// floor(sqrt(I^2 + Q^2)) of int16 codes, exact (the sum is at most 2^31).
+__device__ __forceinline__ uint16_t detect_one(short2 v) {
+    const unsigned s = (unsigned)((int)v.x * v.x) + (unsigned)((int)v.y * v.y);
+    unsigned d = (unsigned)sqrtf((float)s);
+    while (d > 0 && (unsigned long long)d * d > s) --d;
+    while ((unsigned long long)(d + 1) * (d + 1) <= s) ++d;
+    return (uint16_t)min(d, 65535u);
+}
+
+__global__ void k_detect(const short2 *z, int64_t count, uint16_t *out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) out[i] = detect_one(z[i]);
+}
+
+__global__ void k_zero_i32(int *p, int count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = 0;
+}
+
+// fixedpoint.fit_scale's max|re|, max|im| over a complex64 array (float bits of
+// non-negative values order like unsigned integers).
+__global__ void k_absmax(const float *x, int64_t count, unsigned *maxbits) {
+    unsigned m = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (int64_t)gridDim.x * blockDim.x)
+        m = max(m, __float_as_uint(fabsf(x[i])));
+    for (int off = 32; off >= 1; off >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(maxbits, m);
+}
+
+__global__ void k_quantize(const float *x, int64_t count, float lsb, int16_t *out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) out[i] = (int16_t)fminf(fmaxf(floorf(x[i] / lsb), -32768.f), 32767.f);
+}
+
+// ------------------------------------------------------------------ resample helpers
+// 2-tap lerp of serialize_inputs' (idx, wq) contract: a + floor((b-a)*wq / 2^15).
+// f32: the product is rounded to float32 first (form_image's complex64 line).
+__device__ __forceinline__ short2 lerp_q15(const short2 *row, int idx, int wq, bool f32) {
+    const short2 a = row[idx], bb = row[idx + 1];
+    int pr = (bb.x - a.x) * wq, pi = (bb.y - a.y) * wq;
+    if (f32) {
+        pr = (int)__int2float_rn(pr);
+        pi = (int)__int2float_rn(pi);
+    }
+    return make_short2((short)sat16(a.x + (pr >> 15)), (short)sat16(a.y + (pi >> 15)));
+}
+
+// 32-tap polyphase sinc (gen_resample_vectors.gather_sinc): taps idx-15..idx+16,
+// exact integer dot product, round half up, sat16.  Each sample is split into
+// a signed high byte and an unsigned low byte so both partial sums stay in
+// int32 (|c*xh| <= 2^22, |c*xl| <= 2^23 over 32 taps).
+__device__ __forceinline__ short2 sinc_q15(const short2 *row, int idx, const int16_t *taps) {
+    int hr = 0, lr = 0, hi = 0, li = 0;
+    const short2 *s = row + idx - 15;
+#pragma unroll
+    for (int t = 0; t < 32; ++t) {
+        const int c = taps[t];
+        const short2 v = s[t];
+        hr += c * (v.x >> 8);
+        lr += c * (v.x & 255);
+        hi += c * (v.y >> 8);
+        li += c * (v.y & 255);
+    }
+    const int64_t ar = (int64_t)hr * 256 + lr + 16384;
+    const int64_t ai = (int64_t)hi * 256 + li + 16384;
+    return make_short2((short)max(min(ar >> 15, (int64_t)32767), (int64_t)-32768),
+                       (short)max(min(ai >> 15, (int64_t)32767), (int64_t)-32768));
+}
+
+__device__ __forceinline__ int q15_weight(double w) {
+    const double r = rint(w * 32768.0);
+    return (int)fmin(fmax(r, 0.0), 32767.0);
+}
+
+// pass-1 source position x_j = f64(a) * (f64(f0) + j * f64(df)), each op rounded
+// (silicon_emulator.py:144 under NumPy 2 promotion).
+__device__ __forceinline__ double xpos(double a, double f0, double df, int j) {
+    return __dmul_rn(a, __dadd_rn(f0, __dmul_rn((double)j, df)));
+}
+
+// ------------------------------------------------------------------ pass 1
+constexpr int RT = 64;   // pulses per tile
+constexpr int QT = 64;   // range outputs per tile
+
+__global__ __launch_bounds__(256) void k_range_resample(
+    const short2 *__restrict__ sig, int m, int n, int grid, const float *__restrict__ a32,
+    const float *__restrict__ f0v, const float *__restrict__ dfv,
+    const int32_t *__restrict__ order, const float *__restrict__ krp,
+    const int16_t *__restrict__ sinc, int flags, short2 *__restrict__ g1t, int ldp) {
+    __shared__ short2 tile[QT][RT + 1];
+    const int p0 = blockIdx.x * RT, q0 = blockIdx.y * QT;
+    const int t = threadIdx.x;
+    const int pl = t >> 2, ch = t & 3;
+    const int p = p0 + pl;
+    const bool use_sinc = flags & PFA_FIXED_RANGE_SINC;
+    const bool f32 = !(flags & PFA_FIXED_LERP_EXACT);
+    if (p < m) {
+        const int i = order[p];
+        const double a = a32[i], f0 = f0v[i], df = dfv[i];
+        const short2 *row = sig + (size_t)i * n;
+        const bool asc = xpos(a, f0, df, n - 1) >= xpos(a, f0, df, 0);
+        // ascending view xa(u) = x(asc ? u : n-1-u); c = #{xa < Q}
+        auto xa = [&](int u) { return xpos(a, f0, df, asc ? u : n - 1 - u); };
+        int c = -1;
+        for (int u = 0; u < QT / 4; ++u) {
+            const int q = q0 + ch * (QT / 4) + u;
+            const double Q = krp[q];
+            if (c < 0) {                      // closed-form guess
+                const double js = (Q / a - f0) / df;
+                double g = asc ? js : (double)(n - 1) - js;
+                g = fmin(fmax(floor(g) + 1.0, 0.0), (double)n);
+                c = (g == g) ? (int)g : 0;
+            }
+            int steps = 0;
+            while (c < n && xa(c) < Q && steps < 64) { ++c; ++steps; }
+            while (c > 0 && !(xa(c - 1) < Q) && steps < 128) { --c; ++steps; }
+            if (steps >= 64) {                // far from the guess: binary search
+                int lo = 0, hi = n;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (xa(mid) < Q) lo = mid + 1; else hi = mid;
+                }
+                c = lo;
+            }
+            const int k = c - 1;
+            short2 v = make_short2(0, 0);
+            if (k >= 0 && k <= n - 2) {
+                const double x0 = xa(k), x1 = xa(k + 1);
+                const double frac = (Q - x0) / (x1 - x0);
+                const int idx = asc ? k : n - 2 - k;
+                const int wq = q15_weight(asc ? frac : 1.0 - frac);
+                if (use_sinc && idx >= 15 && idx <= n - 17)
+                    v = sinc_q15(row, idx, sinc + (wq >> 7) * 32);
+                else
+                    v = lerp_q15(row, idx, wq, f32 && !use_sinc);
+            }
+            tile[q - q0][pl] = v;
+        }
+    }
+    __syncthreads();
+    for (int r = t; r < QT * RT; r += 256) {
+        const int qq = r / RT, pp = r % RT;
+        if (p0 + pp < m) g1t[(size_t)(q0 + qq) * ldp + p0 + pp] = tile[qq][pp];
+    }
+}
+
+// ------------------------------------------------------------------ pass 2 + window + FFT A
+template <int L, bool DO_FFT>
+__global__ __launch_bounds__(Fft<L>::NB) void k_azimuth_fft(
+    const short2 *__restrict__ g1t, int ldp, int m, const float *__restrict__ krp,
+    const float *__restrict__ tan_s, const float *__restrict__ kcp,
+    const int16_t *__restrict__ sinc, int flags, const int32_t *__restrict__ hamr,
+    const int32_t *__restrict__ hamc, int do_window, short2 *__restrict__ out,
+    int32_t *__restrict__ e_row, int *__restrict__ emax, const short2 *__restrict__ twg) {
+    using F = Fft<L>;
+    constexpr int G = F::N;
+    __shared__ short2 data[F::DATA];
+    __shared__ float srcv[G];            // sources for the bracket search, then twiddles
+    __shared__ int red[2 * F::NW];
+    const int j = blockIdx.x;
+    const int t = threadIdx.x;
+    const float kr = krp[j];
+    for (int p = t; p < m; p += blockDim.x) srcv[p] = __fmul_rn(kr, tan_s[p]);
+    __syncthreads();
+    const bool asc = srcv[m - 1] >= srcv[0];
+    auto sa = [&](int u) { return srcv[asc ? u : m - 1 - u]; };
+    const short2 *row = g1t + (size_t)j * ldp;
+    const bool use_sinc = flags & PFA_FIXED_AZ_SINC;
+    const int hr = hamr[j];
+    constexpr int PER = G / F::NB > 0 ? G / F::NB : 1;
+    int c = -1;
+    for (int u = 0; u < PER; ++u) {
+        const int q = t * PER + u;
+        if (q >= G) break;
+        const float Q = kcp[q];
+        if (c < 0) {
+            int lo = 0, hi = m;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (sa(mid) < Q) lo = mid + 1; else hi = mid;
+            }
+            c = lo;
+        } else {
+            int steps = 0;
+            while (c < m && sa(c) < Q && steps < 64) { ++c; ++steps; }
+            while (c > 0 && !(sa(c - 1) < Q) && steps < 128) { --c; ++steps; }
+            if (steps >= 64) {
+                int lo = 0, hi = m;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (sa(mid) < Q) lo = mid + 1; else hi = mid;
+                }
+                c = lo;
+            }
+        }
+        const int k = c - 1;
+        short2 v = make_short2(0, 0);
+        if (k >= 0 && k <= m - 2) {
+            const double x0 = sa(k), x1 = sa(k + 1);
+            const double frac = ((double)Q - x0) / (x1 - x0);
+            const int idx = asc ? k : m - 2 - k;
+            const int wq = q15_weight(asc ? frac : 1.0 - frac);
+            if (use_sinc && idx >= 15 && idx <= m - 17)
+                v = sinc_q15(row, idx, sinc + (wq >> 7) * 32);
+            else
+                v = lerp_q15(row, idx, wq, false);
+        }
+        if (do_window) {
+            const int cw = wrap16((hr * hamc[q]) >> 15);
+            v = make_short2((short)wrap16((v.x * cw) >> 15), (short)wrap16((v.y * cw) >> 15));
+        }
+        if (DO_FFT) data[pad32(q)] = v;
+        else out[(size_t)j * G + q] = v;
+    }
+    if constexpr (DO_FFT) {
+        __syncthreads();
+        short2 *tw = reinterpret_cast<short2 *>(srcv);
+        load_twiddles<L>(tw, twg);
+        __syncthreads();
+        int xr[F::E], xi[F::E], e;
+        F::run(data, tw, red, xr, xi, e);
+        if (t < F::NT) {
+#pragma unroll
+            for (int l = 0; l < F::E; ++l)
+                out[(size_t)j * G + F::idx(t, l, L - F::P)] =
+                    make_short2((short)xr[l], (short)xi[l]);
+        }
+        if (t == 0) {
+            e_row[j] = e;
+            atomicMax(emax, e);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ FFT B (range axis)
+// Workgroup b handles cross column k; consecutive columns go to the same XCD
+// (workgroups are dispatched round-robin over the 8 XCDs) so their strided
+// column reads share that XCD's L2 lines.
+template <int L>
+__global__ __launch_bounds__(Fft<L>::NB) void k_fft_b(const short2 *__restrict__ ya,
+                                                     const int32_t *__restrict__ ea,
+                                                     const int *__restrict__ emax_a,
+                                                     short2 *__restrict__ yb,
+                                                     int32_t *__restrict__ eb,
+                                                     int *__restrict__ emax_b,
+                                                     const short2 *__restrict__ twg) {
+    using F = Fft<L>;
+    constexpr int G = F::N;
+    __shared__ short2 data[F::DATA];
+    __shared__ short2 tw[G / 2];
+    __shared__ int red[2 * F::NW];
+    const int b = blockIdx.x;
+    const int per_xcd = G / 8;
+    const int k = (G >= 8) ? (b % 8) * per_xcd + b / 8 : b;
+    const int ema = *emax_a;
+    for (int r = threadIdx.x; r < G; r += blockDim.x) {
+        const short2 v = ya[(size_t)r * G + k];
+        const int sh = ema - ea[r];
+        data[pad32(r)] = make_short2((short)sat16(v.x >> sh), (short)sat16(v.y >> sh));
+    }
+    load_twiddles<L>(tw, twg);
+    __syncthreads();
+    int xr[F::E], xi[F::E], e;
+    F::run(data, tw, red, xr, xi, e);
+    if (threadIdx.x < F::NT) {
+#pragma unroll
+        for (int l = 0; l < F::E; ++l)
+            yb[(size_t)k * G + F::idx(threadIdx.x, l, L - F::P)] =
+                make_short2((short)xr[l], (short)xi[l]);
+    }
+    if (threadIdx.x == 0) {
+        eb[k] = e;
+        atomicMax(emax_b, e);
+    }
+}
+
+// pass-B renorm + detect, transposing yb[cross k][range j] to out[range j][cross k].
+__global__ __launch_bounds__(256) void k_detect_t(const short2 *__restrict__ yb,
+                                                  const int32_t *__restrict__ eb,
+                                                  const int *__restrict__ emax_b, int G,
+                                                  uint16_t *__restrict__ out) {
+    __shared__ uint16_t tile[64][65];
+    const int k0 = blockIdx.y * 64, j0 = blockIdx.x * 64;
+    const int emb = *emax_b;
+    for (int r = threadIdx.x; r < 64 * 64; r += 256) {
+        const int kk = r / 64, jj = r % 64;
+        const short2 v = yb[(size_t)(k0 + kk) * G + j0 + jj];
+        const int sh = emb - eb[k0 + kk];
+        tile[jj][kk] = detect_one(make_short2((short)sat16(v.x >> sh), (short)sat16(v.y >> sh)));
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < 64 * 64; r += 256) {
+        const int jj = r / 64, kk = r % 64;
+        out[(size_t)(j0 + jj) * G + k0 + kk] = tile[jj][kk];
+    }
+}
+
+inline int grid1(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
+
+const short2 *twiddles16(pfa_ctx *ctx, int n, int *st) {
+    char slot[32];
+    snprintf(slot, sizeof slot, "tw16_%d", n);
+    auto it = ctx->ws.find(slot);
+    if (it != ctx->ws.end() && it->second.bytes) return (const short2 *)it->second.ptr;
+    std::vector<int32_t> codes((size_t)n);
+    pfa_twiddle_codes(n, 16, codes.data());
+    std::vector<short2> h((size_t)n / 2);
+    for (int k = 0; k < n / 2; ++k) h[k] = make_short2((short)codes[2 * k], (short)codes[2 * k + 1]);
+    void *d = pfa_ws(ctx, slot, h.size() * sizeof(short2), st);
+    if (!d) return nullptr;
+    if (hipMemcpy(d, h.data(), h.size() * sizeof(short2), hipMemcpyHostToDevice) != hipSuccess) {
+        *st = pfa_fail(ctx, PFA_ERR_HIP, "twiddle upload failed");
+        return nullptr;
+    }
+    return (const short2 *)d;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ launchers
+#define PFA_SWITCH_L(L, MACRO) \
+    switch (L) {               \
+        MACRO(6) MACRO(7) MACRO(8) MACRO(9) MACRO(10) MACRO(11) MACRO(12) MACRO(13) \
+        default: return pfa_fail(ctx, PFA_ERR_DIMS, "FFT length must be a power of two in 64..8192"); \
+    }
+
+static int log2_exact(int n) {
+    int l = 0;
+    while ((1 << l) < n) ++l;
+    return (1 << l) == n ? l : -1;
+}
+
+int launch_fft_bfp_rows(pfa_ctx *ctx, const FixedFFTArgs &a) {
+    int st = PFA_OK;
+    const short2 *tw = twiddles16(ctx, a.n, &st);
+    if (!tw) return st;
+    const int L = log2_exact(a.n);
+    if (a.rows <= 0) return PFA_OK;
+#define LAUNCH(LL)                                                                          \
+    case LL:                                                                                \
+        hipLaunchKernelGGL(k_fft_rows<LL>, dim3(a.rows), dim3(Fft<LL>::NB), 0, ctx->stream, \
+                           (const short2 *)a.in, a.in_ld, (short2 *)a.out, a.e_row, a.emax, tw); \
+        break;
+    PFA_SWITCH_L(L, LAUNCH)
+#undef LAUNCH
+    PFA_HIP(ctx, hipGetLastError());
+    return PFA_OK;
+}
+
+int launch_renorm_rows(pfa_ctx *ctx, void *y, const int32_t *e_row, const int *emax, int rows,
+                       int n) {
+    const int64_t total = (int64_t)rows * n;
+    if (total == 0) return PFA_OK;
+    hipLaunchKernelGGL(k_renorm_rows, dim3(grid1(total, 256)), dim3(256), 0, ctx->stream,
+                       (short2 *)y, e_row, emax, n, total);
+    PFA_HIP(ctx, hipGetLastError());
+    return PFA_OK;
+}
+
+int launch_window_fixed(pfa_ctx *ctx, const void *g2, int rows, int cols, const int32_t *hamr,
+                        const int32_t *hamc, void *out) {
+    const int64_t total = (int64_t)rows * cols;
+    if (total == 0) return PFA_OK;
+    hipLaunchKernelGGL(k_window, dim3(grid1(total, 256)), dim3(256), 0, ctx->stream,
+                       (const short2 *)g2, rows, cols, hamr, hamc, (short2 *)out);
+    PFA_HIP(ctx, hipGetLastError());
+    return PFA_OK;
+}
+
+int launch_detect_fixed(pfa_ctx *ctx, const void *z, int64_t count, uint16_t *out) {
+    if (count == 0) return PFA_OK;
+    hipLaunchKernelGGL(k_detect, dim3(grid1(count, 256)), dim3(256), 0, ctx->stream,
+                       (const short2 *)z, count, out);
+    PFA_HIP(ctx, hipGetLastError());
+    return PFA_OK;
+}
+
+int launch_zero_i32(pfa_ctx *ctx, int *p, int count) {
+    hipLaunchKernelGGL(k_zero_i32, dim3(grid1(count, 64)), dim3(64), 0, ctx->stream, p, count);
+    PFA_HIP(ctx, hipGetLastError());
+    return PFA_OK;
+}
+
+int launch_quantize(pfa_ctx *ctx, const float *sig, int64_t count, int16_t *out, int *exp_out) {
+    // count = number of float values (2 per complex sample)
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, ctx->stream, (int *)ctx->d_maxbits, 1);
+    const int blocks = (int)std::min<int64_t>(4096, grid1(count, 256));
+    if (count > 0)
+        hipLaunchKernelGGL(k_absmax, dim3(blocks), dim3(256), 0, ctx->stream, sig, count,
+                           ctx->d_maxbits);
+    PFA_HIP(ctx, hipGetLastError());
+    unsigned bits = 0;
+    PFA_HIP(ctx, hipMemcpyAsync(&bits, ctx->d_maxbits, 4, hipMemcpyDeviceToHost, ctx->stream));
+    PFA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    float mxf;
+    memcpy(&mxf, &bits, 4);
+    const double mx = mxf;
+    int e = 0;
+    if (mx > 0.0) {   // smallest e with mx <= 32767 * 2^e (fixedpoint.fit_scale)
+        e = (int)ceil(log2(mx / 32767.0));
+        while (mx > 32767.0 * ldexp(1.0, e)) ++e;
+        while (mx <= 32767.0 * ldexp(1.0, e - 1)) --e;
+    }
+    if (e < -126 || e > 127)
+        return pfa_fail(ctx, PFA_ERR_GEOMETRY, "signal scale outside the float32 exponent range");
+    *exp_out = e;
+    if (count > 0)
+        hipLaunchKernelGGL(k_quantize, dim3(grid1(count, 256)), dim3(256), 0, ctx->stream, sig,
+                           count, ldexpf(1.0f, e), out);
+    PFA_HIP(ctx, hipGetLastError());
+    return PFA_OK;
+}
+
+int launch_fixed_range(pfa_plan *p, const void *sig, void *g1t, int ldp) {
+    pfa_ctx *ctx = p->ctx;
+    dim3 grid((p->m + RT - 1) / RT, p->grid / QT);
+    pfa_timer_begin(ctx, PFA_TIMER_RANGE);
+    hipLaunchKernelGGL(k_range_resample, grid, dim3(256), 0, ctx->stream, (const short2 *)sig,
+                       p->m, p->n, p->grid, p->a32, p->f0, p->df, p->order, p->krp, p->sinc,
+                       p->flags, (short2 *)g1t, ldp);
+    pfa_timer_end(ctx, PFA_TIMER_RANGE);
+    PFA_HIP(ctx, hipGetLastError());
+    return PFA_OK;
+}
+
+int launch_fixed_azimuth_fft(pfa_plan *p, const void *g1t, int ldp, void *ya, int32_t *ea,
+                             int *emax_a, bool do_fft, bool do_window) {
+    pfa_ctx *ctx = p->ctx;
+    int st = PFA_OK;
+    const short2 *tw = twiddles16(ctx, p->grid, &st);
+    if (!tw) return st;
+    const int L = log2_exact(p->grid);
+    pfa_timer_begin(ctx, PFA_TIMER_AZIMUTH);
+#define LAUNCH(LL)                                                                             \
+    case LL:                                                                                   \
+        if (do_fft)                                                                            \
+            hipLaunchKernelGGL((k_azimuth_fft<LL, true>), dim3(p->grid), dim3(Fft<LL>::NB), 0, \
+                               ctx->stream, (const short2 *)g1t, ldp, p->m, p->krp, p->tan_s,  \
+                               p->kcp, p->sinc, p->flags, p->hamr, p->hamc, (int)do_window,    \
+                               (short2 *)ya, ea, emax_a, tw);                                  \
+        else                                                                                   \
+            hipLaunchKernelGGL((k_azimuth_fft<LL, false>), dim3(p->grid), dim3(Fft<LL>::NB), 0,\
+                               ctx->stream, (const short2 *)g1t, ldp, p->m, p->krp, p->tan_s,  \
+                               p->kcp, p->sinc, p->flags, p->hamr, p->hamc, (int)do_window,    \
+                               (short2 *)ya, ea, emax_a, tw);                                  \
+        break;
+    PFA_SWITCH_L(L, LAUNCH)
+#undef LAUNCH
+    pfa_timer_end(ctx, PFA_TIMER_AZIMUTH);
+    PFA_HIP(ctx, hipGetLastError());
+    return PFA_OK;
+}
+
+int launch_fixed_fft_b(pfa_plan *p, const void *ya, const int32_t *ea, const int *emax_a,
+                       void *yb, int32_t *eb, int *emax_b) {
+    pfa_ctx *ctx = p->ctx;
+    int st = PFA_OK;
+    const short2 *tw = twiddles16(ctx, p->grid, &st);
+    if (!tw) return st;
+    const int L = log2_exact(p->grid);
+    pfa_timer_begin(ctx, PFA_TIMER_FFT_B);
+#define LAUNCH(LL)                                                                           \
+    case LL:                                                                                 \
+        hipLaunchKernelGGL(k_fft_b<LL>, dim3(p->grid), dim3(Fft<LL>::NB), 0, ctx->stream,    \
+                           (const short2 *)ya, ea, emax_a, (short2 *)yb, eb, emax_b, tw);    \
+        break;
+    PFA_SWITCH_L(L, LAUNCH)
+#undef LAUNCH
+    pfa_timer_end(ctx, PFA_TIMER_FFT_B);
+    PFA_HIP(ctx, hipGetLastError());
+    return PFA_OK;
+}
+
+int launch_fixed_detect_t(pfa_plan *p, const void *yb, const int32_t *eb, const int *emax_b,
+                          uint16_t *out) {
+    pfa_ctx *ctx = p->ctx;
+    const int G = p->grid;
+    pfa_timer_begin(ctx, PFA_TIMER_DETECT);
+    hipLaunchKernelGGL(k_detect_t, dim3(G / 64, G / 64), dim3(256), 0, ctx->stream,
+                       (const short2 *)yb, eb, emax_b, G, out);
+    pfa_timer_end(ctx, PFA_TIMER_DETECT);
+    PFA_HIP(ctx, hipGetLastError());
+    return PFA_OK;
+}

@@ -1,0 +1,88 @@
+// pfa_internal.hpp -- context, plan and launcher declarations shared by the
+// host API (pfa_api.cpp) and the kernel translation units (*.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "mi355x_pfa.h"
+
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct TimerMark {
+    int kind;
+    hipEvent_t start, stop;
+};
+
+struct pfa_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::map<std::string, DevBuf> ws;     // grow-only named device workspace
+    bool timing = false;
+    std::vector<TimerMark> marks;
+    std::vector<hipEvent_t> event_pool;
+    int *d_counters = nullptr;            // [0]=emax pass A, [1]=emax pass B, [2..] scratch
+    unsigned *d_maxbits = nullptr;        // quantiser max |x| as float bits
+};
+
+enum PlanKind { PLAN_FIXED = 1, PLAN_FLOAT = 2 };
+
+struct pfa_plan {
+    PlanKind kind;
+    pfa_ctx *ctx;
+    int m = 0, n = 0, grid = 0, flags = 0;
+    // fixed-mode device tables
+    float *a32 = nullptr, *f0 = nullptr, *df = nullptr, *tan_s = nullptr;
+    float *krp = nullptr, *kcp = nullptr;
+    int32_t *order = nullptr, *hamr = nullptr, *hamc = nullptr;
+    int16_t *sinc = nullptr;
+    std::vector<void *> owned;            // device allocations of the plan
+};
+
+// ---------------------------------------------------------------- helpers
+int pfa_fail(pfa_ctx *ctx, int code, const std::string &msg);
+int pfa_hip_fail(pfa_ctx *ctx, hipError_t e, const char *what);
+void *pfa_ws(pfa_ctx *ctx, const char *slot, size_t bytes, int *status);
+void pfa_timer_begin(pfa_ctx *ctx, int kind);
+void pfa_timer_end(pfa_ctx *ctx, int kind);
+
+#define PFA_HIP(ctx, call)                                        \
+    do {                                                          \
+        hipError_t e_ = (call);                                   \
+        if (e_ != hipSuccess) return pfa_hip_fail((ctx), e_, #call); \
+    } while (0)
+
+// ---------------------------------------------------------------- fixed launchers
+struct FixedFFTArgs {
+    const void *in;          // short2 rows (row-major, ld = in_ld)
+    int in_ld;
+    void *out;               // short2 rows (ld = n)
+    int32_t *e_row;
+    int *emax;               // may be null
+    int rows, n;
+};
+int launch_fft_bfp_rows(pfa_ctx *ctx, const FixedFFTArgs &a);
+int launch_renorm_rows(pfa_ctx *ctx, void *y, const int32_t *e_row, const int *emax,
+                       int rows, int n);
+int launch_window_fixed(pfa_ctx *ctx, const void *g2, int rows, int cols,
+                        const int32_t *hamr, const int32_t *hamc, void *out);
+int launch_detect_fixed(pfa_ctx *ctx, const void *z, int64_t count, uint16_t *out);
+int launch_quantize(pfa_ctx *ctx, const float *sig, int64_t count, int16_t *out, int *exp_out);
+int launch_zero_i32(pfa_ctx *ctx, int *p, int count);
+
+int launch_fixed_range(pfa_plan *p, const void *sig, void *g1t, int ldp);
+int launch_fixed_azimuth_fft(pfa_plan *p, const void *g1t, int ldp, void *ya,
+                             int32_t *ea, int *emax_a, bool do_fft, bool do_window);
+int launch_fixed_fft_b(pfa_plan *p, const void *ya, const int32_t *ea, const int *emax_a,
+                       void *yb, int32_t *eb, int *emax_b);
+int launch_fixed_detect_t(pfa_plan *p, const void *yb, const int32_t *eb, const int *emax_b,
+                          uint16_t *out);

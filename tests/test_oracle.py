@@ -192,3 +192,27 @@ def test_sinc32_float_reconstructs_bandlimited_tone():
     assert err.max() < 5e-2          # truncated-sinc ripple + 256-phase delay quantisation
     lin = np.interp(q, xp, fp.real) + 1j * np.interp(q, xp, fp.imag)
     assert np.sqrt(np.mean(err ** 2)) < 0.2 * np.sqrt(np.mean(np.abs(lin - ideal) ** 2))
+
+
+def test_reference_cpp_fft_model_agrees(tmp_path):
+    """oracle/_ref/fft_tb (the reference's fft_tb.cpp, built by oracle/build_ref.py)
+    reproduces the committed vectors from the committed inputs and our twiddles."""
+    from conftest import REPO
+    exe = REPO / "oracle" / "_ref" / "fft_tb"
+    if not exe.exists():
+        pytest.skip("oracle/_ref not built (no reference checkout here)")
+    import subprocess
+    g = load_golden("corefft_vectors.npz")
+    wr, wi = rf.twiddle_codes(8192, 18)
+    (tmp_path / "tw.csv").write_text("\n".join(f"{a},{b}" for a, b in zip(wr, wi)) + "\n")
+    for name in ("impulse_k", "random", "twotone_hidr"):
+        i = g[f"{name}_in"].astype(np.int64)
+        words = ((i[:, 0] & 0xFFFF) << 16) | (i[:, 1] & 0xFFFF)
+        (tmp_path / "in.hex").write_text("\n".join(f"{w:08x}" for w in words) + "\n")
+        subprocess.run([str(exe), "8192", str(tmp_path / "in.hex"), str(tmp_path / "tw.csv"),
+                        str(tmp_path / "out.hex")], check=True, capture_output=True)
+        w = np.array([int(s, 16) for s in (tmp_path / "out.hex").read_text().split()], np.int64)
+        re = (w >> 16) & 0xFFFF; im = w & 0xFFFF
+        re = np.where(re >= 0x8000, re - 0x10000, re); im = np.where(im >= 0x8000, im - 0x10000, im)
+        o = g[f"{name}_out"].astype(np.int64)
+        assert np.array_equal(re, o[:, 0]) and np.array_equal(im, o[:, 1]), name
