@@ -133,6 +133,46 @@ int pfa_form_image_fixed(pfa_plan *plan, const int16_t *sig_iq, uint16_t *out,
 int pfa_form_image_fixed_dev(pfa_plan *plan, const int16_t *sig_iq, uint16_t *out,
                              int32_t *exp_a, int32_t *exp_b);
 
+/* ------------------------------------------------------------------ float mode
+ * form_image_pfa.py in FP32.  The O(m + n) scene geometry (form_image_pfa.py
+ * :178-193, 205-209: rotated frame, pr, tan(phi), pulse order, KR/KC grids) is
+ * prepared by the caller exactly as the reference computes it with numpy; the
+ * O(m n) k-space positions kr = (2 f / c) pr and every resample position are
+ * computed on the GPU in FP64.
+ *   pr[m]          per-pulse range projection (float64)
+ *   order[m]       argsort(tan_phi) (pulse index of each sorted position)
+ *   tan_s[m]       tan_phi[order] (float64, ascending)
+ *   kr_grid[n]     KR (np.linspace over the union extent, finding :190-193)
+ *   kc_grid[m]     KC
+ *   f0, df [m]     compact frequencies: freq[i][k] = f0[i] + k * df[i] (each
+ *                  operation rounded, as sc0 + k*scss builds them); both NULL
+ *                  when the full freq[m][n] array is passed at execution
+ *   ham_c[m], ham_r[n]  np.hamming(m), np.hamming(n) (NULL: no window)
+ *   sinc_tab       256x32 Q15 table, needed with PFA_FLOAT_SINC32
+ *   sgn            <0: fft2 (SGN=-1), >0: ifft2 (transform2d :144-147)
+ * The image is the zero-padded (to_pow2) centred transform, [cross][range]. */
+#define PFA_FLOAT_SINC32 0x1        /* 32-tap polyphase gather both passes     */
+enum { PFA_OUT_C64 = 0, PFA_OUT_ABS_F32 = 1 };
+int pfa_plan_float(pfa_ctx *ctx, pfa_plan **plan, int m, int n, int sgn,
+                   const double *pr, const int32_t *order, const double *tan_s,
+                   const double *kr_grid, const double *kc_grid,
+                   const double *f0, const double *df,
+                   const double *ham_c, const double *ham_r,
+                   const int16_t *sinc_tab, int flags);
+/* pfa (form_image_pfa.py:243-246) / NumpyBackend.focus (accel.py:39-44):
+ * sig complex64 [m][n] -> out [m2][n2] complex64 (PFA_OUT_C64) or |img|
+ * float32 (PFA_OUT_ABS_F32).  freq: full float64 [m][n] or NULL (compact). */
+int pfa_focus_f32(pfa_plan *plan, const float *sig_c64, const double *freq, void *out,
+                  int out_kind);
+int pfa_focus_f32_dev(pfa_plan *plan, const float *sig_c64, const double *freq, void *out,
+                      int out_kind);
+/* resample_kspace (form_image_pfa.py:165-222): windowed k-space g2
+ * complex64 [m][n] (rows cross, columns range). */
+int pfa_resample_kspace_f32(pfa_plan *plan, const float *sig_c64, const double *freq,
+                            float *g2_c64);
+/* focus (form_image_pfa.py:225-240) of a given g2 complex64 [m][n]. */
+int pfa_transform_f32(pfa_plan *plan, const float *g2_c64, void *out, int out_kind);
+
 void pfa_plan_destroy(pfa_plan *plan);
 
 #ifdef __cplusplus

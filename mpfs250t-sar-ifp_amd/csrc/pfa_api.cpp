@@ -475,3 +475,158 @@ extern "C" int pfa_form_image_fixed(pfa_plan *p, const int16_t *sig, uint16_t *o
     PFA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return PFA_OK;
 }
+
+// ------------------------------------------------------------------ float: plan + pipeline
+static int pow2_ceil(int v) {
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+extern "C" int pfa_plan_float(pfa_ctx *ctx, pfa_plan **out, int m, int n, int sgn,
+                              const double *pr, const int32_t *order, const double *tan_s,
+                              const double *kr_grid, const double *kc_grid, const double *f0,
+                              const double *df, const double *ham_c, const double *ham_r,
+                              const int16_t *sinc_tab, int flags) {
+    if (!ctx || !out) return PFA_ERR_ARG;
+    *out = nullptr;
+    if (!pr || !order || !tan_s || !kr_grid || !kc_grid)
+        return pfa_fail(ctx, PFA_ERR_ARG, "plan_float: null table");
+    if ((f0 == nullptr) != (df == nullptr)) return pfa_fail(ctx, PFA_ERR_ARG, "plan_float: give both f0 and df or neither");
+    if ((ham_c == nullptr) != (ham_r == nullptr)) return pfa_fail(ctx, PFA_ERR_ARG, "plan_float: give both tapers or neither");
+    if (flags & ~PFA_FLOAT_SINC32) return pfa_fail(ctx, PFA_ERR_ARG, "plan_float: unknown flag");
+    if ((flags & PFA_FLOAT_SINC32) && !sinc_tab) return pfa_fail(ctx, PFA_ERR_ARG, "plan_float: SINC32 needs the tap table");
+    if (sgn == 0) return pfa_fail(ctx, PFA_ERR_ARG, "plan_float: sgn must be -1 or +1");
+    if (m < 2 || n < 2) return pfa_fail(ctx, PFA_ERR_DIMS, "plan_float: need m, n >= 2");
+    const int m2 = std::max(64, pow2_ceil(m)), n2 = std::max(64, pow2_ceil(n));
+    if (m2 > 8192 || n2 > 8192) return pfa_fail(ctx, PFA_ERR_DIMS, "plan_float: padded grid above 8192");
+    if (pow2_ceil(m) != m2 || pow2_ceil(n) != n2)
+        return pfa_fail(ctx, PFA_ERR_DIMS, "plan_float: m and n must be at least 33 (padded size >= 64)");
+    std::vector<int> seen(m, 0);
+    for (int i = 0; i < m; ++i) {
+        if (order[i] < 0 || order[i] >= m || seen[order[i]]++) return pfa_fail(ctx, PFA_ERR_GEOMETRY, "order is not a permutation");
+        if (!std::isfinite(pr[i]) || pr[i] == 0.0 || !std::isfinite(tan_s[i]) || !std::isfinite(kc_grid[i]))
+            return pfa_fail(ctx, PFA_ERR_GEOMETRY, "plan_float: non-finite or zero per-pulse geometry");
+        if (i && tan_s[i] < tan_s[i - 1]) return pfa_fail(ctx, PFA_ERR_GEOMETRY, "tan_s must be ascending");
+        if (f0 && (!std::isfinite(f0[i]) || !std::isfinite(df[i]))) return pfa_fail(ctx, PFA_ERR_GEOMETRY, "plan_float: non-finite frequencies");
+    }
+    for (int j = 0; j < n; ++j)
+        if (!std::isfinite(kr_grid[j]) || kr_grid[j] <= 0.0) return pfa_fail(ctx, PFA_ERR_GEOMETRY, "KR must be finite and positive");
+    pfa_plan *p = new (std::nothrow) pfa_plan();
+    if (!p) return PFA_ERR_NOMEM;
+    p->kind = PLAN_FLOAT;
+    p->ctx = ctx;
+    p->m = m; p->n = n; p->m2 = m2; p->n2 = n2; p->sgn = sgn < 0 ? -1 : 1; p->flags = flags;
+    int st = PFA_OK;
+    bool ok = (p->pr64 = upload(p, pr, m, &st)) && (p->order = upload(p, order, m, &st)) &&
+              (p->tan_s64 = upload(p, tan_s, m, &st)) && (p->KR = upload(p, kr_grid, n, &st)) &&
+              (p->KC = upload(p, kc_grid, m, &st));
+    if (ok && f0) ok = (p->f0d = upload(p, f0, m, &st)) && (p->dfd = upload(p, df, m, &st));
+    if (ok && ham_c) ok = (p->hm = upload(p, ham_c, m, &st)) && (p->hn = upload(p, ham_r, n, &st));
+    if (ok && sinc_tab) ok = (p->sinc = upload(p, sinc_tab, 256 * 32, &st)) != nullptr;
+    if (!ok) {
+        pfa_plan_destroy(p);
+        return st;
+    }
+    *out = p;
+    return PFA_OK;
+}
+
+static int ldp32(int m) { return (m + 31) / 32 * 32; }
+
+static int float_check(pfa_plan *p, const void *sig, const double *freq, const void *out, int out_kind) {
+    if (!p || p->kind != PLAN_FLOAT) return PFA_ERR_STATE;
+    if (!sig || !out) return pfa_fail(p->ctx, PFA_ERR_ARG, "float: null pointer");
+    if (out_kind != PFA_OUT_C64 && out_kind != PFA_OUT_ABS_F32) return pfa_fail(p->ctx, PFA_ERR_ARG, "float: bad out_kind");
+    if (!freq && !p->f0d) return pfa_fail(p->ctx, PFA_ERR_ARG, "float: plan has no compact frequencies; pass freq");
+    return PFA_OK;
+}
+
+extern "C" int pfa_focus_f32_dev(pfa_plan *p, const float *sig, const double *freq, void *out,
+                                 int out_kind) {
+    CK(float_check(p, sig, freq, out, out_kind));
+    pfa_ctx *ctx = p->ctx;
+    int st = PFA_OK;
+    const int ldp = ldp32(p->m);
+    void *g1t = pfa_ws(ctx, "f_g1t", (size_t)p->n * ldp * 8, &st);
+    if (!g1t) return st;
+    void *A = pfa_ws(ctx, "f_A", (size_t)p->n * p->m2 * 8, &st);
+    if (!A) return st;
+    CK(launch_float_range(p, sig, freq, g1t, ldp));
+    CK(launch_float_azimuth(p, 1, g1t, ldp, nullptr, A));
+    CK(launch_float_fft_b(p, A, out, out_kind));
+    return PFA_OK;
+}
+
+extern "C" int pfa_focus_f32(pfa_plan *p, const float *sig, const double *freq, void *out,
+                             int out_kind) {
+    CK(float_check(p, sig, freq, out, out_kind));
+    pfa_ctx *ctx = p->ctx;
+    int st = PFA_OK;
+    const size_t in_b = (size_t)p->m * p->n * 8;
+    const size_t out_b = (size_t)p->m2 * p->n2 * (out_kind == PFA_OUT_C64 ? 8 : 4);
+    void *ds = pfa_ws(ctx, "io_in", in_b, &st);
+    if (!ds) return st;
+    void *dout = pfa_ws(ctx, "io_out", out_b, &st);
+    if (!dout) return st;
+    double *dfreq = nullptr;
+    if (freq) {
+        dfreq = (double *)pfa_ws(ctx, "io_freq", (size_t)p->m * p->n * 8, &st);
+        if (!dfreq) return st;
+        PFA_HIP(ctx, hipMemcpyAsync(dfreq, freq, (size_t)p->m * p->n * 8, hipMemcpyHostToDevice, ctx->stream));
+    }
+    PFA_HIP(ctx, hipMemcpyAsync(ds, sig, in_b, hipMemcpyHostToDevice, ctx->stream));
+    CK(pfa_focus_f32_dev(p, (const float *)ds, dfreq, dout, out_kind));
+    PFA_HIP(ctx, hipMemcpyAsync(out, dout, out_b, hipMemcpyDeviceToHost, ctx->stream));
+    PFA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PFA_OK;
+}
+
+extern "C" int pfa_resample_kspace_f32(pfa_plan *p, const float *sig, const double *freq,
+                                       float *g2) {
+    CK(float_check(p, sig, freq, g2, PFA_OUT_C64));
+    pfa_ctx *ctx = p->ctx;
+    int st = PFA_OK;
+    const size_t in_b = (size_t)p->m * p->n * 8;
+    void *ds = pfa_ws(ctx, "io_in", in_b, &st);
+    if (!ds) return st;
+    void *dg = pfa_ws(ctx, "io_out", in_b, &st);
+    if (!dg) return st;
+    double *dfreq = nullptr;
+    if (freq) {
+        dfreq = (double *)pfa_ws(ctx, "io_freq", (size_t)p->m * p->n * 8, &st);
+        if (!dfreq) return st;
+        PFA_HIP(ctx, hipMemcpyAsync(dfreq, freq, (size_t)p->m * p->n * 8, hipMemcpyHostToDevice, ctx->stream));
+    }
+    const int ldp = ldp32(p->m);
+    void *g1t = pfa_ws(ctx, "f_g1t", (size_t)p->n * ldp * 8, &st);
+    if (!g1t) return st;
+    PFA_HIP(ctx, hipMemcpyAsync(ds, sig, in_b, hipMemcpyHostToDevice, ctx->stream));
+    CK(launch_float_range(p, ds, dfreq, g1t, ldp));
+    CK(launch_float_azimuth(p, 0, g1t, ldp, nullptr, dg));
+    PFA_HIP(ctx, hipMemcpyAsync(g2, dg, in_b, hipMemcpyDeviceToHost, ctx->stream));
+    PFA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PFA_OK;
+}
+
+extern "C" int pfa_transform_f32(pfa_plan *p, const float *g2, void *out, int out_kind) {
+    if (!p || p->kind != PLAN_FLOAT) return PFA_ERR_STATE;
+    pfa_ctx *ctx = p->ctx;
+    if (!g2 || !out || (out_kind != PFA_OUT_C64 && out_kind != PFA_OUT_ABS_F32))
+        return pfa_fail(ctx, PFA_ERR_ARG, "transform: bad argument");
+    int st = PFA_OK;
+    const size_t in_b = (size_t)p->m * p->n * 8;
+    const size_t out_b = (size_t)p->m2 * p->n2 * (out_kind == PFA_OUT_C64 ? 8 : 4);
+    void *dg = pfa_ws(ctx, "io_in", in_b, &st);
+    if (!dg) return st;
+    void *dout = pfa_ws(ctx, "io_out", out_b, &st);
+    if (!dout) return st;
+    void *A = pfa_ws(ctx, "f_A", (size_t)p->n * p->m2 * 8, &st);
+    if (!A) return st;
+    PFA_HIP(ctx, hipMemcpyAsync(dg, g2, in_b, hipMemcpyHostToDevice, ctx->stream));
+    CK(launch_float_azimuth(p, 2, nullptr, 0, dg, A));
+    CK(launch_float_fft_b(p, A, dout, out_kind));
+    PFA_HIP(ctx, hipMemcpyAsync(out, dout, out_b, hipMemcpyDeviceToHost, ctx->stream));
+    PFA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return PFA_OK;
+}

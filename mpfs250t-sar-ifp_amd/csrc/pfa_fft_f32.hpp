@@ -1,0 +1,173 @@
+// pfa_fft_f32.hpp -- LDS-resident FP32 FFT for one line of N = 2^L points.
+//
+// Stockham autosort passes of radix R <= 32 held in registers: each thread
+// reads its R-point butterflies from the LDS line, applies the inter-pass
+// twiddles W_{Ls R}^{j r} (a float2 table of exp(-2 pi i t / N) computed in
+// double on the host), runs the R-point DFT in registers and writes the
+// results back to the same LDS line after a barrier.  The last pass may hand
+// its outputs to a caller-provided sink instead of LDS.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace pfa32 {
+
+constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v >> 1); }
+
+// exp(-2 pi i k / 32) for k < 16, as (cos, sin) of +2 pi k / 32
+__device__ constexpr float kC32[16][2] = {
+    {1.0f, 0.0f},
+    {0.9807852804032304f, 0.19509032201612825f},
+    {0.9238795325112867f, 0.3826834323650898f},
+    {0.8314696123025452f, 0.5555702330196022f},
+    {0.7071067811865476f, 0.7071067811865475f},
+    {0.5555702330196023f, 0.8314696123025452f},
+    {0.38268343236508984f, 0.9238795325112867f},
+    {0.19509032201612833f, 0.9807852804032304f},
+    {0.0f, 1.0f},
+    {-0.1950903220161282f, 0.9807852804032304f},
+    {-0.3826834323650897f, 0.9238795325112867f},
+    {-0.555570233019602f, 0.8314696123025455f},
+    {-0.7071067811865475f, 0.7071067811865476f},
+    {-0.8314696123025453f, 0.5555702330196022f},
+    {-0.9238795325112867f, 0.3826834323650899f},
+    {-0.9807852804032304f, 0.1950903220161286f},
+};
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+// In-register R-point DFT (R = 2^p <= 32), forward sign exp(-i...) or inverse.
+template <int R, bool INV>
+__device__ __forceinline__ void dft(float2 (&x)[R]) {
+    constexpr int P = ilog2c(R);
+    // bit-reverse the order (compile-time swaps)
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        int r = 0;
+#pragma unroll
+        for (int b = 0; b < P; ++b) r |= ((i >> b) & 1) << (P - 1 - b);
+        if (r > i) {
+            const float2 t = x[i];
+            x[i] = x[r];
+            x[r] = t;
+        }
+    }
+#pragma unroll
+    for (int s = 1; s <= P; ++s) {
+        const int m = 1 << s, h = m >> 1;
+#pragma unroll
+        for (int base = 0; base < R; base += m) {
+#pragma unroll
+            for (int j = 0; j < h; ++j) {
+                const float2 a = x[base + j], b = x[base + j + h];
+                float2 t;
+                if (j == 0) {
+                    t = b;
+                } else if (4 * j == m) {          // W = -i (forward) or +i (inverse)
+                    t = INV ? make_float2(-b.y, b.x) : make_float2(b.y, -b.x);
+                } else {
+                    const int k = j * (32 / m);
+                    const float c = kC32[k][0], sn = INV ? kC32[k][1] : -kC32[k][1];
+                    t = make_float2(fmaf(b.x, c, -b.y * sn), fmaf(b.x, sn, b.y * c));
+                }
+                x[base + j] = cadd(a, t);
+                x[base + j + h] = csub(a, t);
+            }
+        }
+    }
+}
+
+template <int L>
+struct Cfg {
+    static constexpr int N = 1 << L;
+    static constexpr int NT_RAW = (N / 32 > 64) ? N / 32 : (N / 2 < 64 ? N / 2 : 64);
+    static constexpr int NT = NT_RAW > 256 ? 256 : NT_RAW;
+    static constexpr int E = N / NT;                  // elements per thread
+    static constexpr int PADN = N + N / 32;           // padded float2 slots
+    // radix of pass k: min(E, remaining)
+    static constexpr int radix(int k) {
+        int rem = N, r = 1;
+        for (int i = 0; i <= k; ++i) {
+            r = rem < E ? rem : E;
+            rem /= r;
+        }
+        return r;
+    }
+    static constexpr int passes() {
+        int rem = N, c = 0;
+        while (rem > 1) {
+            const int r = rem < E ? rem : E;
+            rem /= r;
+            ++c;
+        }
+        return c;
+    }
+};
+
+__device__ __forceinline__ int padf(int i) { return i + (i >> 5); }
+
+// One Stockham pass over the LDS line `buf`; LS = product of earlier radices.
+// On the last pass the outputs go to sink(index, value) instead of LDS.
+template <int L, int R, int LS, bool INV, bool LAST, typename Sink>
+__device__ __forceinline__ void pass(float2 *buf, const float2 *__restrict__ tw, Sink &sink) {
+    using C = Cfg<L>;
+    constexpr int N = C::N, NT = C::NT, B = C::E / R;   // butterflies per thread
+    const int t = threadIdx.x;
+    float2 x[B][R];
+    if (t < NT) {
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const int k = t + b * NT;
+#pragma unroll
+            for (int r = 0; r < R; ++r) x[b][r] = buf[padf(k + r * (N / R))];
+        }
+    }
+    __syncthreads();
+    if (t < NT) {
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const int k = t + b * NT;
+            const int j = k % LS;
+            if (LS > 1) {
+                constexpr int step = N / (LS * R);
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    float2 w = tw[j * r * step];
+                    if (INV) w.y = -w.y;
+                    x[b][r] = cmul(x[b][r], w);
+                }
+            }
+            dft<R, INV>(x[b]);
+            const int o = (k / LS) * LS * R + j;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (LAST) sink(o + r * LS, x[b][r]);
+                else buf[padf(o + r * LS)] = x[b][r];
+            }
+        }
+    }
+    __syncthreads();
+}
+
+template <int L, bool INV, int K, int LS, typename Sink>
+__device__ __forceinline__ void fft_from(float2 *buf, const float2 *__restrict__ tw, Sink &sink) {
+    using C = Cfg<L>;
+    if constexpr (K < C::passes()) {
+        constexpr int R = C::radix(K);
+        pass<L, R, LS, INV, K == C::passes() - 1>(buf, tw, sink);
+        fft_from<L, INV, K + 1, LS * R>(buf, tw, sink);
+    }
+}
+
+// Full transform of the padded natural-order line in `buf`; the last pass
+// delivers (index, value) to `sink` (which may write back to buf).
+template <int L, bool INV, typename Sink>
+__device__ __forceinline__ void fft(float2 *buf, const float2 *__restrict__ tw, Sink &&sink) {
+    fft_from<L, INV, 0, 1>(buf, tw, sink);
+}
+
+}  // namespace pfa32

@@ -45,6 +45,10 @@ struct pfa_plan {
     float *krp = nullptr, *kcp = nullptr;
     int32_t *order = nullptr, *hamr = nullptr, *hamc = nullptr;
     int16_t *sinc = nullptr;
+    // float-mode device tables
+    int sgn = -1, m2 = 0, n2 = 0;
+    double *pr64 = nullptr, *tan_s64 = nullptr, *KR = nullptr, *KC = nullptr;
+    double *f0d = nullptr, *dfd = nullptr, *hm = nullptr, *hn = nullptr;
     std::vector<void *> owned;            // device allocations of the plan
 };
 
@@ -86,3 +90,9 @@ int launch_fixed_fft_b(pfa_plan *p, const void *ya, const int32_t *ea, const int
                        void *yb, int32_t *eb, int *emax_b);
 int launch_fixed_detect_t(pfa_plan *p, const void *yb, const int32_t *eb, const int *emax_b,
                           uint16_t *out);
+
+// ---------------------------------------------------------------- float launchers
+int launch_float_range(pfa_plan *p, const void *sig, const double *freq, void *g1t, int ldp);
+int launch_float_azimuth(pfa_plan *p, int mode, const void *g1t, int ldp, const void *g2in,
+                         void *out);
+int launch_float_fft_b(pfa_plan *p, const void *A, void *out, int out_kind);

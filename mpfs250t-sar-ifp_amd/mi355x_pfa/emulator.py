@@ -47,15 +47,18 @@ class FixedPlan:
         flags = ((_lib.FIXED_RANGE_SINC if range_sinc else 0)
                  | (_lib.FIXED_AZ_SINC if az_sinc else 0)
                  | (_lib.FIXED_LERP_EXACT if lerp_exact else 0))
-        self._keep = {k: np.ascontiguousarray(v) for k, v in tabs.items()}
+        types = {"inv_order": np.int32, "hamr": np.int32, "hamc": np.int32}
+        # keep every table alive (and C-contiguous, right dtype) for the call
+        self._keep = {k: np.ascontiguousarray(v, types.get(k, np.float32))
+                      for k, v in tabs.items()}
         sinc = sinc_table_q15() if (range_sinc or az_sinc) else None
         self._sinc = None if sinc is None else np.ascontiguousarray(sinc)
         k = self._keep
         self._h = C.c_void_p()
         self.ctx.call("pfa_plan_fixed", C.byref(self._h), m, n, grid,
                       ptr(k["f0"]), ptr(k["df"]), ptr(k["pr"]), ptr(k["tan_s"]),
-                      ptr(k["inv_order"].astype(np.int32)), ptr(k["krp"]), ptr(k["kcp"]),
-                      ptr(k["hamr"].astype(np.int32)), ptr(k["hamc"].astype(np.int32)),
+                      ptr(k["inv_order"]), ptr(k["krp"]), ptr(k["kcp"]),
+                      ptr(k["hamr"]), ptr(k["hamc"]),
                       ptr(self._sinc), flags)
 
     def resample(self, sig_iq: np.ndarray) -> np.ndarray:
@@ -158,9 +161,9 @@ def fft_bfp_perrow(x, device: int = 0):
 
 def detect_fixed(z, device: int = 0):
     """silicon_emulator.detect_fixed -> uint16."""
-    iq = _to_iq(z)
+    iq = np.ascontiguousarray(_to_iq(z))
     out = np.empty(iq.shape[:-1], np.uint16)
-    context(device).call("pfa_detect_fixed", ptr(np.ascontiguousarray(iq)), out.size, ptr(out))
+    context(device).call("pfa_detect_fixed", ptr(iq), out.size, ptr(out))
     return out
 
 

@@ -1,0 +1,111 @@
+"""Float mode on the GPU against form_image_pfa.py.
+
+Tolerance (the float contract): peak-normalised RMS error
+    rms(img_gpu - img_ref) / max|img_ref| <= 1e-5
+on the complex image, with identical point-target peak positions.  The
+reference fixtures come from the reference itself; larger and SINC32 cases use
+the oracle (oracle/ref_float.py, pinned to those fixtures)."""
+import numpy as np
+import pytest
+
+from conftest import load_golden
+from oracle import ref_float as rfl
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def pn_rms(a, b):
+    return float(np.sqrt(np.mean(np.abs(a.astype(np.complex128) - b) ** 2)) / np.max(np.abs(b)))
+
+
+def top_peaks(img, k, guard=3):
+    a = np.abs(img).astype(np.float64).copy()
+    out = []
+    for _ in range(k):
+        r, c = np.unravel_index(np.argmax(a), a.shape)
+        out.append((int(r), int(c)))
+        a[max(r - guard, 0):r + guard + 1, max(c - guard, 0):c + guard + 1] = 0
+    return out
+
+
+@pytest.fixture(scope="module")
+def gp():
+    from mi355x_pfa import pfa
+    return pfa
+
+
+@pytest.mark.parametrize("name", ["small", "rect"])
+def test_resample_kspace_matches_reference(gp, name):
+    g = load_golden(f"float_{name}.npz")
+    print(f"resample_kspace {name}", flush=True)
+    g2, geo = gp.resample_kspace(g["sig"], g["freq"], g["ax"], g["ay"])
+    ref = g["g2"]
+    err = np.max(np.abs(g2.astype(np.complex128) - ref)) / np.max(np.abs(ref))
+    assert err <= 1e-6, err
+    assert np.allclose([geo["dc"], geo["dr"], *geo["dhat"], *geo["chat"]][:2],
+                       [g["geo"][0] * 1.0, g["geo"][1] * 1.0], rtol=1.0)   # unpadded here
+
+
+@pytest.mark.parametrize("name", ["small", "rect"])
+def test_focus_and_pfa_match_reference(gp, name):
+    g = load_golden(f"float_{name}.npz")
+    sgn = int(g["sgn"])
+    print(f"focus/pfa {name}", flush=True)
+    img, geo = gp.focus(g["g2"], {"dc": 1.0, "dr": 1.0, "dhat": (1, 0), "chat": (0, 1)}, sgn)
+    assert pn_rms(img, g["img"]) <= TOL
+    img2, geo2 = gp.pfa(g["sig"], g["freq"], g["ax"], g["ay"], sgn)
+    assert pn_rms(img2, g["img"]) <= TOL
+    assert np.allclose([geo2["dc"], geo2["dr"], *geo2["dhat"], *geo2["chat"]], g["geo"],
+                       rtol=1e-15, atol=0)
+    det = gp.GpuBackend().focus(g["sig"], {"freq": g["freq"], "ax": g["ax"], "ay": g["ay"],
+                                           "sgn": sgn})
+    assert det.dtype == np.float32 and det.shape == g["det"].shape
+    assert float(np.sqrt(np.mean((det - g["det"]) ** 2)) / g["det"].max()) <= TOL
+    assert top_peaks(img2, 3) == top_peaks(g["img"], 3)
+
+
+def _scene(m, n, seed, targets=16, half=None, desc=False):
+    from mi355x_pfa import synth
+    sc = synth.make_scene(synth.SceneSpec(m, n, targets=targets, seed=seed,
+                                          scene_half=half or 0.2 * min(m, n)))
+    freq = sc.freq()
+    if desc:
+        freq = freq[:, ::-1].copy()
+    ax, ay = sc.look()
+    return sc, freq, ax, ay
+
+
+@pytest.mark.parametrize("m,n,sgn,desc", [(1024, 1024, -1, False), (700, 1100, 1, False),
+                                          (300, 257, -1, True)])
+def test_linear_pfa_matches_oracle(gp, m, n, sgn, desc):
+    sc, freq, ax, ay = _scene(m, n, seed=m + n, desc=desc)
+    print(f"linear pfa {m}x{n} sgn={sgn} desc={desc}", flush=True)
+    img, _ = gp.pfa(sc.sig, freq, ax, ay, sgn)
+    ref, _ = rfl.pfa(sc.sig, freq, ax, ay, sgn)
+    assert pn_rms(img, ref) <= TOL
+    assert top_peaks(img, 8) == top_peaks(ref, 8)
+
+
+@pytest.mark.parametrize("m,n", [(256, 300), (1024, 1024)])
+def test_sinc32_pfa_matches_fp64_oracle(gp, m, n):
+    sc, freq, ax, ay = _scene(m, n, seed=3 * m + n)
+    print(f"sinc32 pfa {m}x{n}", flush=True)
+    img, _ = gp.pfa(sc.sig, freq, ax, ay, -1, interp="sinc32")
+    ref, _ = rfl.pfa(sc.sig, freq, ax, ay, -1, interp="sinc32", store=np.complex128)
+    assert pn_rms(img, ref) <= TOL
+    assert top_peaks(img, 8) == top_peaks(ref, 8)
+
+
+def test_compact_frequencies_equal_full_array(gp):
+    sc, freq, ax, ay = _scene(200, 180, seed=9)
+    m, n = freq.shape
+    geom_full = gp.float_geometry(ax, ay, freq=freq)
+    geom_c = gp.float_geometry(ax, ay, f0=sc.sc0, df=sc.scss, n=n)
+    for k in ("KR", "KC", "tan_s", "order", "pr"):
+        assert np.array_equal(geom_full[k], geom_c[k]), k
+    p_full = gp.FloatPlan(m, n, -1, geom_full)
+    p_c = gp.FloatPlan(m, n, -1, geom_c, f0=sc.sc0, df=sc.scss)
+    a = p_full.focus(sc.sig, freq)
+    b = p_c.focus(sc.sig)
+    assert np.array_equal(a, b)
