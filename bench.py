@@ -1,0 +1,264 @@
+"""Benchmark: ms per 8192^2 Polar Format frame on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config C3] [--interp sinc32]
+                    [--no-fixed] [--no-cpu] [--copy-calibration]
+
+Workload (default): 8167 pulses x 8192 samples -> 8192^2 grid, 32-tap sinc in
+both resample passes, FP32, synthetic spotlight phase history (256 point
+targets + 512 exponential-power clutter scatterers, 20 dB SNR) synthesised on
+the GPU.  A step is one full focus of the frame (range resample + corner-turn,
+azimuth resample + window + FFT along cross, FFT along range + detect) with the
+phase history already resident in HBM.  The fixed-point (bit-exact) frame on the
+same scene is timed as well.  Per-kernel device times come from hipEvents the
+library records on the stream its kernels run on.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO))
+sys.path.insert(0, str(REPO / "mpfs250t-sar-ifp_amd"))
+
+import numpy as np  # noqa: E402
+
+METRIC = "ms per 8192² PFA frame (FP32 vs BF16-MFMA FFT @ stated error); HBM GB/s"
+PEAK_HBM = 8.0e12            # MI355X published HBM3E bandwidth, bytes/s
+FPGA_FRAME_MS = 15738.0      # reference silicon, 8167x8192 -> 8192^2 fixed, sinc both passes
+
+
+def cpu_info():
+    model = platform.processor() or ""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "cpu_count": os.cpu_count(),
+            "affinity": len(os.sched_getaffinity(0))}
+
+
+def make_inputs(spec, device):
+    from mi355x_pfa import synth
+    from mi355x_pfa.pfa import float_geometry
+    sc = synth.make_scene(spec, device=device)
+    ax, ay = sc.look()
+    geom = float_geometry(ax, ay, f0=sc.sc0, df=sc.scss, n=spec.samples)
+    return sc, ax, ay, geom
+
+
+def algorithmic_bytes_float(m, n, m2, n2, s_in=8, s_mid=8, s_out=4):
+    """Compulsory HBM bytes per kernel of the 3-kernel float staging."""
+    return {"range": s_in * m * n + s_mid * n * m,          # read phase history, write g1t
+            "azimuth_fft": s_mid * n * m + s_mid * n * m2,  # read g1t, write A
+            "fft_b_detect": s_mid * n * m2 + s_out * m2 * n2}
+
+
+def time_float(plan, sig, out, steps, warmup, torch):
+    from mi355x_pfa import _lib
+    ctx = plan.ctx
+    for _ in range(warmup):
+        plan.focus_dev(sig, out, detect=True)
+    torch.cuda.synchronize()
+    ctx.timer_reset()
+    ctx.timer_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.focus_dev(sig, out, detect=True)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps * 1e3
+    kern = {}
+    for name, kind in (("range", _lib.TIMER_RANGE), ("azimuth_fft", _lib.TIMER_AZIMUTH),
+                       ("fft_b_detect", _lib.TIMER_FFT_B)):
+        ms, cnt = ctx.timer_read(kind)
+        kern[name] = ms / max(cnt, 1)
+    ctx.timer_enable(False)
+    ctx.timer_reset()
+    return wall, kern
+
+
+def time_fixed(sc, ax, ay, geom, steps, warmup, torch):
+    from mi355x_pfa import _lib
+    from mi355x_pfa.emulator import FixedPlan
+    from mi355x_pfa.tables import fixed_tables
+    import ctypes as C
+    spec = sc.spec
+    m, n, G = spec.pulses, spec.samples, 8192
+    freq2 = np.stack([sc.sc0, sc.sc0 + sc.scss], axis=1)
+    tabs = fixed_tables({"ax": ax, "ay": ay, "freq": np.pad(freq2, ((0, 0), (0, n - 2))),
+                         "KR": geom["KR"], "KC": geom["KC"], "tan_phi": geom["tan_phi"]}, m, n, G)
+    plan = FixedPlan(tabs, m, n, G, True, True)
+    ctx = plan.ctx
+    sig_iq = torch.empty((m, n, 2), dtype=torch.int16, device="cuda")
+    e = C.c_int(0)
+    ctx.check(_lib.lib().pfa_quantize_iq16_dev(ctx.handle, sc.sig.data_ptr(), m * n,
+                                               sig_iq.data_ptr(), C.byref(e)), "quantize")
+    out = torch.empty((G, G), dtype=torch.int16, device="cuda")
+    for _ in range(warmup):
+        plan.form_image_dev(sig_iq, out)
+    torch.cuda.synchronize()
+    ctx.timer_reset()
+    ctx.timer_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.form_image_dev(sig_iq, out)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps * 1e3
+    kern = {}
+    for name, kind in (("range", _lib.TIMER_RANGE), ("azimuth_fft", _lib.TIMER_AZIMUTH),
+                       ("fft_b", _lib.TIMER_FFT_B), ("detect", _lib.TIMER_DETECT)):
+        ms, cnt = ctx.timer_read(kind)
+        kern[name] = ms / max(cnt, 1)
+    ctx.timer_enable(False)
+    ctx.timer_reset()
+    plan.close()
+    del sig_iq, out
+    return wall, kern
+
+
+def cpu_baseline(spec, interp, geom, budget_rows=160):
+    """The numpy oracle on the host, on a bounded sample of the frame: pass 1 on
+    `budget_rows` pulses and pass 2 on `budget_rows` range bins of the real
+    geometry (each scaled to the full frame), plus the full 2-D FFT + detect."""
+    from oracle import ref_float as rfl
+    from oracle import ref_fixed as rf
+    m, n = spec.pulses, spec.samples
+    rs = np.random.RandomState(0)
+    table = rf.sinc_table_q15()
+    row = (rs.standard_normal(n) + 1j * rs.standard_normal(n)).astype(np.complex64)
+    col = (rs.standard_normal(m) + 1j * rs.standard_normal(m)).astype(np.complex64)
+    f = (spec.f0 - spec.bandwidth / 2) + np.arange(n) * (spec.bandwidth / (n - 1))
+    KR, KC, pr, tan_s = geom["KR"], geom["KC"], geom["pr"], geom["tan_s"]
+    t0 = time.perf_counter()
+    for i in range(budget_rows):
+        kr_i = (2.0 * f / rfl.C_LIGHT) * pr[(i * 7) % m]
+        rfl._interp_line(KR, kr_i, row, interp, table)
+    t1 = time.perf_counter()
+    for j in range(budget_rows):
+        rfl._interp_line(KC, KR[(j * 11) % n] * tan_s, col, interp, table)
+    t2 = time.perf_counter()
+    pad = np.zeros((8192, 8192), np.complex64)
+    pad[:256, :n] = row
+    t3 = time.perf_counter()
+    img = np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(pad)))
+    np.abs(img).astype(np.float32)
+    t4 = time.perf_counter()
+    p1 = (t1 - t0) * m / budget_rows
+    p2 = (t2 - t1) * n / budget_rows
+    total = p1 + p2 + (t4 - t3)
+    return {"value": round(total * 1e3, 1), "unit": "ms/frame", "cores": 1, "kind": "port",
+            "sample": (f"oracle/ref_float {interp}: pass 1 on {budget_rows} of {m} pulses and "
+                       f"pass 2 on {budget_rows} of {n} range bins of this frame's geometry, "
+                       f"each scaled to the frame, plus the full 8192^2 numpy FFT + detect "
+                       f"(numpy interp/fft run single-threaded)"),
+            "measured_s": round(t2 - t0 + t4 - t3, 2),
+            "split_ms": {"pass1": round(p1 * 1e3), "pass2": round(p2 * 1e3),
+                         "fft_detect": round((t4 - t3) * 1e3)}, **cpu_info()}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="C3", choices=["C1", "C2", "C3"])
+    ap.add_argument("--interp", default="sinc32", choices=["sinc32", "linear"])
+    ap.add_argument("--no-fixed", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--quick-scene", action="store_true",
+                    help="16 targets, no clutter (same shapes; for profiler runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 512 MiB device copy (counter calibration)")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one MI355X: the focus path is built and timed on a single "
+              "GPU (multi-GPU sharding is a design note only); run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from mi355x_pfa import synth
+    from mi355x_pfa.pfa import FloatPlan
+    from mi355x_pfa._lib import context
+
+    spec = {"C1": synth.C1, "C2": synth.C2, "C3": synth.C3}[a.config]
+    if a.quick_scene:
+        import dataclasses
+        spec = dataclasses.replace(spec, targets=16, clutter=0)
+    m, n = spec.pulses, spec.samples
+    print(f"[bench] synthesising {a.config} {m}x{n} on the GPU", file=sys.stderr, flush=True)
+    sc, ax, ay, geom = make_inputs(spec, "cuda")
+    ctx = context(0)
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    plan = FloatPlan(m, n, -1, geom, f0=sc.sc0, df=sc.scss, window=True, interp=a.interp)
+    out = torch.empty((plan.m2, plan.n2), dtype=torch.float32, device="cuda")
+    print("[bench] float frames", file=sys.stderr, flush=True)
+    ctx.fallback_counts(reset=True)
+    wall, kern = time_float(plan, sc.sig, out, a.steps, a.warmup, torch)
+    fb = ctx.fallback_counts(reset=True)
+    bytes_k = algorithmic_bytes_float(m, n, plan.m2, plan.n2)
+    frame_dev = sum(kern.values())
+    B3 = sum(bytes_k.values())
+    dom = max(kern, key=kern.get)
+    achieved = bytes_k[dom] / (kern[dom] * 1e-3)
+    traffic = None
+    tfile = REPO / "profiles" / f"traffic_{a.config.lower()}_{a.interp}.json"
+    if tfile.exists():
+        try:
+            traffic = json.loads(tfile.read_text()).get(dom)
+        except Exception:
+            traffic = None
+    res = {
+        "metric": METRIC, "value": round(wall, 4), "unit": "ms/frame",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(wall, 4),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (GPU-synthesised spotlight phase history)",
+        "config": {"workload": f"{a.config}: {m} pulses x {n} samples -> "
+                               f"{plan.m2}x{plan.n2} PFA frame, {a.interp} both passes, FP32",
+                   "pulses": m, "samples": n, "grid": plan.m2, "interp": a.interp,
+                   "staging": "B3 (3 kernels)"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1),
+                     "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM, 4), "traffic": traffic,
+                     "kernel": dom, "bytes_per_launch": bytes_k[dom]},
+        "frame_device_ms": round(frame_dev, 4),
+        "frame_hbm_gbps_B3": round(B3 / (frame_dev * 1e-3) / 1e9, 1),
+        "frame_roofline_frac_B3": round(B3 / (frame_dev * 1e-3) / PEAK_HBM, 4),
+        "kernels_ms": {k: round(v, 4) for k, v in kern.items()},
+        "kernels_gbps": {k: round(bytes_k[k] / (kern[k] * 1e-3) / 1e9, 1) for k in kern},
+        "fallback_outputs_per_frame": [round(c / (a.steps + a.warmup)) for c in fb],
+    }
+    del out
+    plan.close()
+    if a.copy_calibration:
+        x = torch.empty(512 << 20, dtype=torch.uint8, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        res["copy_calibration_bytes"] = 512 << 20
+        del x, y
+    if not a.no_fixed and a.config == "C3":
+        print("[bench] fixed-point frames", file=sys.stderr, flush=True)
+        fwall, fk = time_fixed(sc, ax, ay, geom, max(3, a.steps // 2), 2, torch)
+        res["fixed"] = {"ms_per_frame": round(fwall, 3),
+                        "kernels_ms": {k: round(v, 4) for k, v in fk.items()},
+                        "speedup_vs_fpga_silicon": round(FPGA_FRAME_MS / fwall, 1)}
+    if not a.no_cpu:
+        print("[bench] cpu baseline (numpy oracle, bounded sample)", file=sys.stderr, flush=True)
+        res["cpu_baseline"] = cpu_baseline(spec, a.interp, geom)
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

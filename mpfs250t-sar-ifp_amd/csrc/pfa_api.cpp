@@ -84,8 +84,12 @@ extern "C" int pfa_create(pfa_ctx **out, int device) {
         return PFA_ERR_HIP;
     }
     ctx->stream = ctx->own_stream;
+    int cu = 0;
+    if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cu > 0)
+        ctx->num_cu = cu;
     if (hipMalloc(&ctx->d_counters, 64 * sizeof(int)) != hipSuccess ||
-        hipMalloc(&ctx->d_maxbits, 64 * sizeof(unsigned)) != hipSuccess) {
+        hipMalloc(&ctx->d_maxbits, 64 * sizeof(unsigned)) != hipSuccess ||
+        hipMemset(ctx->d_counters, 0, 64 * sizeof(int)) != hipSuccess) {
         hipStreamDestroy(ctx->own_stream);
         delete ctx;
         return PFA_ERR_NOMEM;
@@ -158,6 +162,14 @@ extern "C" int pfa_timer_read(pfa_ctx *ctx, int kind, float *ms_total, int *laun
     }
     *ms_total = tot;
     *launches = cnt;
+    return PFA_OK;
+}
+
+extern "C" int pfa_fallback_counts(pfa_ctx *ctx, uint32_t out[2], int reset) {
+    if (!ctx || !out) return PFA_ERR_ARG;
+    PFA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    PFA_HIP(ctx, hipMemcpy(out, ctx->d_counters + 32, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (reset) PFA_HIP(ctx, hipMemset(ctx->d_counters + 32, 0, 2 * sizeof(uint32_t)));
     return PFA_OK;
 }
 
@@ -477,6 +489,8 @@ extern "C" int pfa_form_image_fixed(pfa_plan *p, const int16_t *sig, uint16_t *o
 }
 
 // ------------------------------------------------------------------ float: plan + pipeline
+static bool linspace_params(const double *g, int cnt, double out[3]);
+
 static int pow2_ceil(int v) {
     int p = 1;
     while (p < v) p <<= 1;
@@ -510,15 +524,22 @@ extern "C" int pfa_plan_float(pfa_ctx *ctx, pfa_plan **out, int m, int n, int sg
         if (i && tan_s[i] < tan_s[i - 1]) return pfa_fail(ctx, PFA_ERR_GEOMETRY, "tan_s must be ascending");
         if (f0 && (!std::isfinite(f0[i]) || !std::isfinite(df[i]))) return pfa_fail(ctx, PFA_ERR_GEOMETRY, "plan_float: non-finite frequencies");
     }
-    for (int j = 0; j < n; ++j)
+    for (int j = 0; j < n; ++j) {
         if (!std::isfinite(kr_grid[j]) || kr_grid[j] <= 0.0) return pfa_fail(ctx, PFA_ERR_GEOMETRY, "KR must be finite and positive");
+        if (j && kr_grid[j] < kr_grid[j - 1]) return pfa_fail(ctx, PFA_ERR_GEOMETRY, "KR must be non-decreasing");
+    }
+    for (int i = 1; i < m; ++i)
+        if (kc_grid[i] < kc_grid[i - 1]) return pfa_fail(ctx, PFA_ERR_GEOMETRY, "KC must be non-decreasing");
     pfa_plan *p = new (std::nothrow) pfa_plan();
     if (!p) return PFA_ERR_NOMEM;
     p->kind = PLAN_FLOAT;
     p->ctx = ctx;
     p->m = m; p->n = n; p->m2 = m2; p->n2 = n2; p->sgn = sgn < 0 ? -1 : 1; p->flags = flags;
     int st = PFA_OK;
-    bool ok = (p->pr64 = upload(p, pr, m, &st)) && (p->order = upload(p, order, m, &st)) &&
+    std::vector<double> a64(m);               // 2 pr / c per pulse (range positions)
+    for (int i = 0; i < m; ++i) a64[i] = (2.0 * pr[i]) / 299792458.0;
+    bool ok = (p->pr64 = upload(p, pr, m, &st)) && (p->a64 = upload(p, a64.data(), m, &st)) &&
+              (p->order = upload(p, order, m, &st)) &&
               (p->tan_s64 = upload(p, tan_s, m, &st)) && (p->KR = upload(p, kr_grid, n, &st)) &&
               (p->KC = upload(p, kc_grid, m, &st));
     if (ok && f0) ok = (p->f0d = upload(p, f0, m, &st)) && (p->dfd = upload(p, df, m, &st));
@@ -528,11 +549,30 @@ extern "C" int pfa_plan_float(pfa_ctx *ctx, pfa_plan **out, int m, int n, int sg
         pfa_plan_destroy(p);
         return st;
     }
+    if (linspace_params(kr_grid, n, p->kr_lin_v)) p->kr_lin = p->kr_lin_v;
+    if (linspace_params(kc_grid, m, p->kc_lin_v)) p->kc_lin = p->kc_lin_v;
     *out = p;
     return PFA_OK;
 }
 
 static int ldp32(int m) { return (m + 31) / 32 * 32; }
+
+// Is g[0..cnt) exactly np.linspace(g[0], g[cnt-1], cnt)?  Then {start, step, stop}.
+static bool linspace_params(const double *g, int cnt, double out[3]) {
+    if (cnt < 2) return false;
+    const double start = g[0], stop = g[cnt - 1];
+    const volatile double delta = stop - start;
+    const double step = delta / (double)(cnt - 1);
+    if (step == 0.0) return false;
+    for (int k = 0; k < cnt - 1; ++k) {
+        const volatile double prod = (double)k * step;
+        if (prod + start != g[k]) return false;
+    }
+    out[0] = start;
+    out[1] = step;
+    out[2] = stop;
+    return true;
+}
 
 static int float_check(pfa_plan *p, const void *sig, const double *freq, const void *out, int out_kind) {
     if (!p || p->kind != PLAN_FLOAT) return PFA_ERR_STATE;
@@ -550,7 +590,7 @@ extern "C" int pfa_focus_f32_dev(pfa_plan *p, const float *sig, const double *fr
     const int ldp = ldp32(p->m);
     void *g1t = pfa_ws(ctx, "f_g1t", (size_t)p->n * ldp * 8, &st);
     if (!g1t) return st;
-    void *A = pfa_ws(ctx, "f_A", (size_t)p->n * p->m2 * 8, &st);
+    void *A = pfa_ws(ctx, "f_A", (size_t)(p->n + 15) / 16 * 16 * p->m2 * 8, &st);
     if (!A) return st;
     CK(launch_float_range(p, sig, freq, g1t, ldp));
     CK(launch_float_azimuth(p, 1, g1t, ldp, nullptr, A));
@@ -621,7 +661,7 @@ extern "C" int pfa_transform_f32(pfa_plan *p, const float *g2, void *out, int ou
     if (!dg) return st;
     void *dout = pfa_ws(ctx, "io_out", out_b, &st);
     if (!dout) return st;
-    void *A = pfa_ws(ctx, "f_A", (size_t)p->n * p->m2 * 8, &st);
+    void *A = pfa_ws(ctx, "f_A", (size_t)(p->n + 15) / 16 * 16 * p->m2 * 8, &st);
     if (!A) return st;
     PFA_HIP(ctx, hipMemcpyAsync(dg, g2, in_b, hipMemcpyHostToDevice, ctx->stream));
     CK(launch_float_azimuth(p, 2, nullptr, 0, dg, A));

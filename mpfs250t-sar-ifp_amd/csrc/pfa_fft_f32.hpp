@@ -2,9 +2,9 @@
 //
 // Stockham autosort passes of radix R <= 32 held in registers: each thread
 // reads its R-point butterflies from the LDS line, applies the inter-pass
-// twiddles W_{Ls R}^{j r} (a float2 table of exp(-2 pi i t / N) computed in
-// double on the host), runs the R-point DFT in registers and writes the
-// results back to the same LDS line after a barrier.  The last pass may hand
+// twiddles W_{Ls R}^{j r} (two-level tables in LDS, see Tw), runs the R-point
+// DFT in registers and writes the results back to the same LDS line after a
+// barrier.  The last pass may hand
 // its outputs to a caller-provided sink instead of LDS.
 #pragma once
 
@@ -81,11 +81,14 @@ __device__ __forceinline__ void dft(float2 (&x)[R]) {
     }
 }
 
-template <int L>
+// Line of N = 2^L points held by NT threads with E = N / NT elements each
+// (E <= EMAX, NT >= min(64, N/2)).
+template <int L, int EMAX = 32>
 struct Cfg {
     static constexpr int N = 1 << L;
-    static constexpr int NT_RAW = (N / 32 > 64) ? N / 32 : (N / 2 < 64 ? N / 2 : 64);
-    static constexpr int NT = NT_RAW > 256 ? 256 : NT_RAW;
+    static constexpr int NT_RAW = (N / EMAX > 64) ? N / EMAX : (N / 2 < 64 ? N / 2 : 64);
+    static constexpr int NT = NT_RAW;
+    static constexpr int NB = NT < 64 ? 64 : NT;      // block size
     static constexpr int E = N / NT;                  // elements per thread
     static constexpr int PADN = N + N / 32;           // padded float2 slots
     // radix of pass k: min(E, remaining)
@@ -110,11 +113,26 @@ struct Cfg {
 
 __device__ __forceinline__ int padf(int i) { return i + (i >> 5); }
 
+// Twiddle exp(-2 pi i t / N) from two small tables staged in LDS:
+// hi[t >> 6] = exp(-2 pi i 64 (t >> 6) / N), lo[t & 63] = exp(-2 pi i (t & 63) / N),
+// both rounded from double on the host; one complex product per twiddle.
+struct Tw {
+    const float2 *hi, *lo;
+    __device__ __forceinline__ float2 operator()(int t) const { return cmul(hi[t >> 6], lo[t & 63]); }
+};
+
+// Stage the two twiddle tables of an N-point line (N / 64 + 64 float2).
+template <int N>
+__device__ __forceinline__ Tw load_tw(float2 *lds, const float2 *__restrict__ g) {
+    for (int i = threadIdx.x; i < N / 64 + 64; i += blockDim.x) lds[i] = g[i];
+    return Tw{lds, lds + N / 64};
+}
+
 // One Stockham pass over the LDS line `buf`; LS = product of earlier radices.
 // On the last pass the outputs go to sink(index, value) instead of LDS.
-template <int L, int R, int LS, bool INV, bool LAST, typename Sink>
-__device__ __forceinline__ void pass(float2 *buf, const float2 *__restrict__ tw, Sink &sink) {
-    using C = Cfg<L>;
+template <int L, int EMAX, int R, int LS, bool INV, bool LAST, typename Src, typename Sink>
+__device__ __forceinline__ void pass(float2 *buf, const Tw &tw, Src *src, Sink &sink) {
+    using C = Cfg<L, EMAX>;
     constexpr int N = C::N, NT = C::NT, B = C::E / R;   // butterflies per thread
     const int t = threadIdx.x;
     float2 x[B][R];
@@ -123,10 +141,11 @@ __device__ __forceinline__ void pass(float2 *buf, const float2 *__restrict__ tw,
         for (int b = 0; b < B; ++b) {
             const int k = t + b * NT;
 #pragma unroll
-            for (int r = 0; r < R; ++r) x[b][r] = buf[padf(k + r * (N / R))];
+            for (int r = 0; r < R; ++r)
+                x[b][r] = src ? (*src)(k + r * (N / R)) : buf[padf(k + r * (N / R))];
         }
     }
-    __syncthreads();
+    if (!src) __syncthreads();
     if (t < NT) {
 #pragma unroll
         for (int b = 0; b < B; ++b) {
@@ -136,7 +155,7 @@ __device__ __forceinline__ void pass(float2 *buf, const float2 *__restrict__ tw,
                 constexpr int step = N / (LS * R);
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
-                    float2 w = tw[j * r * step];
+                    float2 w = tw(j * r * step);
                     if (INV) w.y = -w.y;
                     x[b][r] = cmul(x[b][r], w);
                 }
@@ -153,21 +172,32 @@ __device__ __forceinline__ void pass(float2 *buf, const float2 *__restrict__ tw,
     __syncthreads();
 }
 
-template <int L, bool INV, int K, int LS, typename Sink>
-__device__ __forceinline__ void fft_from(float2 *buf, const float2 *__restrict__ tw, Sink &sink) {
-    using C = Cfg<L>;
+template <int L, int EMAX, bool INV, int K, int LS, typename Src, typename Sink>
+__device__ __forceinline__ void fft_from(float2 *buf, const Tw &tw, Src *src, Sink &sink) {
+    using C = Cfg<L, EMAX>;
     if constexpr (K < C::passes()) {
         constexpr int R = C::radix(K);
-        pass<L, R, LS, INV, K == C::passes() - 1>(buf, tw, sink);
-        fft_from<L, INV, K + 1, LS * R>(buf, tw, sink);
+        pass<L, EMAX, R, LS, INV, K == C::passes() - 1>(buf, tw, K == 0 ? src : (Src *)nullptr, sink);
+        fft_from<L, EMAX, INV, K + 1, LS * R>(buf, tw, src, sink);
     }
 }
 
+struct NoSrc {
+    __device__ float2 operator()(int) const { return make_float2(0.f, 0.f); }
+};
+
 // Full transform of the padded natural-order line in `buf`; the last pass
 // delivers (index, value) to `sink` (which may write back to buf).
-template <int L, bool INV, typename Sink>
-__device__ __forceinline__ void fft(float2 *buf, const float2 *__restrict__ tw, Sink &&sink) {
-    fft_from<L, INV, 0, 1>(buf, tw, sink);
+template <int L, bool INV, int EMAX = 32, typename Sink>
+__device__ __forceinline__ void fft(float2 *buf, const Tw &tw, Sink &&sink) {
+    fft_from<L, EMAX, INV, 0, 1>(buf, tw, (NoSrc *)nullptr, sink);
+}
+
+// As fft, but the first pass loads element e with src(e) (e.g. from global
+// memory) instead of from the LDS line.
+template <int L, bool INV, int EMAX = 32, typename Src, typename Sink>
+__device__ __forceinline__ void fft_src(float2 *buf, const Tw &tw, Src &&src, Sink &&sink) {
+    fft_from<L, EMAX, INV, 0, 1>(buf, tw, &src, sink);
 }
 
 }  // namespace pfa32

@@ -23,6 +23,7 @@ struct TimerMark {
 
 struct pfa_ctx {
     int device = 0;
+    int num_cu = 256;                     // compute units of the device
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     std::string err;
@@ -47,8 +48,12 @@ struct pfa_plan {
     int16_t *sinc = nullptr;
     // float-mode device tables
     int sgn = -1, m2 = 0, n2 = 0;
-    double *pr64 = nullptr, *tan_s64 = nullptr, *KR = nullptr, *KC = nullptr;
+    double *pr64 = nullptr, *a64 = nullptr, *tan_s64 = nullptr, *KR = nullptr, *KC = nullptr;
     double *f0d = nullptr, *dfd = nullptr, *hm = nullptr, *hn = nullptr;
+    // KR / KC as np.linspace parameters {start, step, stop} when they are
+    // exactly such grids (the kernels then form them in registers)
+    double kr_lin_v[3] = {0, 0, 0}, kc_lin_v[3] = {0, 0, 0};
+    const double *kr_lin = nullptr, *kc_lin = nullptr;
     std::vector<void *> owned;            // device allocations of the plan
 };
 

@@ -44,6 +44,7 @@ PROTOS = {
     "pfa_timer_enable": [_vp, _i],
     "pfa_timer_read": [_vp, _i, C.POINTER(C.c_float), C.POINTER(_i)],
     "pfa_timer_reset": [_vp],
+    "pfa_fallback_counts": [_vp, _vp, _i],
     "pfa_twiddle_codes": [_i, _i, _vp],
     "pfa_quantize_iq16": [_vp, _vp, _i64, _vp, C.POINTER(_i)],
     "pfa_quantize_iq16_dev": [_vp, _vp, _i64, _vp, C.POINTER(_i)],
@@ -140,6 +141,11 @@ class Context:
         n = _i(0)
         self.call("pfa_timer_read", int(kind), C.byref(ms), C.byref(n))
         return ms.value, n.value
+
+    def fallback_counts(self, reset=True):
+        out = np.zeros(2, np.uint32)
+        self.call("pfa_fallback_counts", out.ctypes.data, int(reset))
+        return int(out[0]), int(out[1])
 
     def close(self):
         if self._h:
