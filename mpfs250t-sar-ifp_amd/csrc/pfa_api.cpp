@@ -545,6 +545,22 @@ extern "C" int pfa_plan_float(pfa_ctx *ctx, pfa_plan **out, int m, int n, int sg
     if (ok && f0) ok = (p->f0d = upload(p, f0, m, &st)) && (p->dfd = upload(p, df, m, &st));
     if (ok && ham_c) ok = (p->hm = upload(p, ham_c, m, &st)) && (p->hn = upload(p, ham_r, n, &st));
     if (ok && sinc_tab) ok = (p->sinc = upload(p, sinc_tab, 256 * 32, &st)) != nullptr;
+    if (ok) {                                 // tan bins: 2 per pulse over [tan_s[0], tan_s[m-1]]
+        const double lo = tan_s[0], hi = tan_s[m - 1];
+        int bins = 2 * m;
+        double inv_w = (hi > lo) ? (double)bins / (hi - lo) : 0.0;
+        if (!(inv_w > 0.0) || !std::isfinite(inv_w)) { bins = 1; inv_w = 0.0; }
+        std::vector<int32_t> tb(bins + 1);
+        for (int b = 0, c = 0; b <= bins; ++b) {
+            const double edge = inv_w > 0.0 ? lo + (double)b / inv_w : (b ? hi : lo);
+            while (c < m && tan_s[c] < edge) ++c;
+            tb[b] = c;
+        }
+        p->tb_bins = bins;
+        p->tb_lo = lo;
+        p->tb_inv_w = inv_w;
+        ok = (p->tan_bin = upload(p, tb.data(), tb.size(), &st)) != nullptr;
+    }
     if (!ok) {
         pfa_plan_destroy(p);
         return st;

@@ -134,7 +134,10 @@ template <int L, int EMAX, int R, int LS, bool INV, bool LAST, typename Src, typ
 __device__ __forceinline__ void pass(float2 *buf, const Tw &tw, Src *src, Sink &sink) {
     using C = Cfg<L, EMAX>;
     constexpr int N = C::N, NT = C::NT, B = C::E / R;   // butterflies per thread
-    const int t = threadIdx.x;
+    // opaque per pass: in a loop over lines, the twiddles and indices would
+    // otherwise be hoisted and held in registers across the whole loop
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
     float2 x[B][R];
     if (t < NT) {
 #pragma unroll

@@ -70,199 +70,62 @@ __device__ __forceinline__ void load_taps(float *tapl, const int16_t *__restrict
     }
 }
 
-// LDS index padding for sample strips: one spare float2 per 32, so per-thread
-// windows that start 8, 16 or 32 samples apart do not share banks.
-__device__ __forceinline__ int spad(int i) { return i + (i >> 5); }
-
 constexpr int KD = 4;            // bracket advance covered by one 4-output window
 constexpr int KW = 32 + KD;      // samples in that window
 
 // Four consecutive outputs whose brackets J_u = J_0 + d_u (0 <= d_u <= KD) share
-// one register window w[k] = fa(J_0 - 15 + k); output u uses taps shifted by d_u:
-// sum_k tap[ph_u][k - d_u] * w[k].  Each window sample is read once per four
+// one window w[k] = row[J_0 - 15 + k], k < KW; output u uses taps shifted by
+// d_u: sum_k tap[ph_u][k - d_u] w[k].  Each window sample is read once per four
 // outputs instead of once per output.
-// PADDED: samples in an spad-padded LDS strip; otherwise a plain array (global).
-template <bool PADDED = true>
-__device__ __forceinline__ void sinc4(const float2 *strip, int start, const float *tapl,
-                                      const int (&ph)[4], const int (&d)[4], float2 (&out)[4]) {
+// The row is an unpadded LDS array: every window read is an LDS load with an
+// immediate offset, so the loop is pure loads and FMAs.
+template <int BATCH = 6>
+__device__ __forceinline__ void sinc4u(const float2 *row, int start, const float *tapl,
+                                       const int (&ph)[4], const int (&d)[4], float2 (&out)[4]) {
     const float *col[4];
     float ar[4], ai[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-        col[u] = tapl + (4 - d[u]) * 256 + scr(ph[u]);   // col[u][k * 256] = tap k - d[u]
+        col[u] = tapl + (4 - d[u]) * 256 + scr(ph[u]);
         ar[u] = 0.f;
         ai[u] = 0.f;
     }
-    const float2 *ws = strip + (PADDED ? spad(start) : start);
-    const int brk = 32 - (start & 31);             // window indices where the strip padding steps
+    const float2 *ws = row + start;
+    static_assert(KW % BATCH == 0, "window splits into batches");
+    // a rolled loop over batches: fully unrolled, the compiler front-loads all
+    // 180 window and tap loads and runs out of registers
+#pragma unroll 1
+    for (int kb = 0; kb < KW; kb += BATCH) {
 #pragma unroll
-    for (int k = 0; k < KW; ++k) {                 // one window sample feeds four outputs
-        const float2 w = PADDED ? ws[k + (k >= brk ? 1 : 0) + (k >= brk + 32 ? 1 : 0)] : ws[k];
+        for (int k = 0; k < BATCH; ++k) {
+            const float2 w = ws[k];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const float c = col[u][k * 256];
-            ar[u] = fmaf(c, w.x, ar[u]);
-            ai[u] = fmaf(c, w.y, ai[u]);
+            for (int u = 0; u < 4; ++u) {
+                const float c = col[u][k * 256];
+                ar[u] = fmaf(c, w.x, ar[u]);
+                ai[u] = fmaf(c, w.y, ai[u]);
+            }
         }
-        if (k % 12 == 11) __builtin_amdgcn_sched_barrier(0);     // bound live loads
+        ws += BATCH;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) col[u] += BATCH * 256;
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) out[u] = make_float2(ar[u], ai[u]);
 }
 
-// np.interp / SINC32 value of one output from its bracket state.
-// cls: 0 zero, 1 value fa(J) (exact hit or last sample), 2 linear, 3 sinc interior
-struct Bracket {
-    int cls, J, ph;
-    double x0, x1;
-};
-
-__device__ __forceinline__ Bracket classify(bool sinc, int c, int S, double Q, double xprev,
-                                            double xcur, double xlast) {
-    Bracket b{0, 0, 0, 0.0, 0.0};
-    if (c <= 0) return b;
-    if (c >= S) {
-        if (Q == xlast) { b.cls = 1; b.J = S - 1; }
-        return b;
-    }
-    b.J = c - 1;
-    b.x0 = xprev;
-    b.x1 = xcur;
-    if (sinc && b.J >= 15 && b.J <= S - 17) {
-        const double frac = __ddiv_rn(__dsub_rn(Q, xprev), __dsub_rn(xcur, xprev));
-        const int wq = (int)fmin(fmax(rint(frac * 32768.0), 0.0), 32767.0);
-        b.cls = 3;
-        b.ph = wq >> 7;
-        return b;
-    }
-    b.cls = (xprev == Q) ? 1 : 2;
-    return b;
-}
-
+// SINC32 value of one interior output (bracket J, phase ph) read through fa.
 template <typename FA>
-__device__ __forceinline__ float2 value_slow(const Bracket &b, double Q, FA &&fa, const float *tapl) {
-    if (b.cls == 0) return make_float2(0.f, 0.f);
-    if (b.cls == 1) return fa(b.J);
-    if (b.cls == 3) {
-        const float *col = tapl + 4 * 256 + scr(b.ph);
-        float ar = 0.f, ai = 0.f;
+__device__ __forceinline__ float2 sinc1(FA &&fa, int J, int ph, const float *tapl) {
+    const float *col = tapl + 4 * 256 + scr(ph);
+    float ar = 0.f, ai = 0.f;
 #pragma unroll 8
-        for (int t = 0; t < 32; ++t) {
-            const float2 s = fa(b.J - 15 + t);
-            ar = fmaf(col[t * 256], s.x, ar);
-            ai = fmaf(col[t * 256], s.y, ai);
-        }
-        return make_float2(ar, ai);
+    for (int t = 0; t < 32; ++t) {
+        const float2 s = fa(J - 15 + t);
+        ar = fmaf(col[t * 256], s.x, ar);
+        ai = fmaf(col[t * 256], s.y, ai);
     }
-    const float2 f0 = fa(b.J), f1 = fa(b.J + 1);
-    const double dx = __dsub_rn(b.x1, b.x0), dq = __dsub_rn(Q, b.x0);
-    const double sr = __ddiv_rn(__dsub_rn((double)f1.x, (double)f0.x), dx);
-    const double si = __ddiv_rn(__dsub_rn((double)f1.y, (double)f0.y), dx);
-    return make_float2((float)__dadd_rn(__dmul_rn(sr, dq), (double)f0.x),
-                       (float)__dadd_rn(__dmul_rn(si, dq), (double)f0.y));
-}
-
-// Value of one output from its scattered bracket e = (J << 8) | ph.
-template <bool SINC, typename FA>
-__device__ __forceinline__ float2 value_scattered(int e, int S, double Q, double x0, double x1,
-                                                  FA &&fa, const float *tapl) {
-    Bracket b{0, e >> 8, e & 255, x0, x1};
-    b.cls = (SINC && b.J >= 15 && b.J <= S - 17) ? 3 : (x0 == Q ? 1 : 2);
-    return value_slow(b, Q, fa, tapl);
-}
-
-// Bracket walk over an ascending source view xa(0..S-1) for non-decreasing
-// queries: c = #{xa <= Q}, with xa(c-1) and xa(c) cached so each output costs
-// about one position evaluation.
-template <typename XA>
-struct Walk {
-    XA &xa;
-    int S;
-    int c = -1;
-    double xprev = -INFINITY, xcur = INFINITY;
-
-    __device__ Walk(XA &f, int s) : xa(f), S(s) {}
-
-    __device__ __forceinline__ void set(int c0) {
-        c = c0;
-        xprev = c > 0 ? xa(c - 1) : -INFINITY;
-        xcur = c < S ? xa(c) : INFINITY;
-    }
-    __device__ __forceinline__ void seek(double Q, int guess) {
-        if (c < 0) {
-            int lo = 0, hi = S;
-            if (guess >= 0) {                 // verify a closed-form guess, else bisect
-                const int g = min(max(guess, 0), S);
-                const bool left_ok = (g == 0) || xa(g - 1) <= Q;
-                const bool right_ok = (g == S) || !(xa(g) <= Q);
-                if (left_ok && right_ok) { set(g); return; }
-                if (left_ok) lo = g; else hi = g;
-            }
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (xa(mid) <= Q) lo = mid + 1; else hi = mid;
-            }
-            set(lo);
-            return;
-        }
-        while (c < S && xcur <= Q) {
-            ++c;
-            xprev = xcur;
-            xcur = c < S ? xa(c) : INFINITY;
-        }
-    }
-    // Start from the bracket c0 of a neighbouring line (positions moved a
-    // little): walk both ways from it, bisect if it is far off.
-    __device__ __forceinline__ void start_near(double Q, int c0) {
-        c = -1;
-        if (c0 < 0) { seek(Q, -1); return; }
-        int c1 = min(max(c0, 0), S), steps = 0;
-        while (c1 > 0 && !(xa(c1 - 1) <= Q) && steps < 48) { --c1; ++steps; }
-        while (c1 < S && xa(c1) <= Q && steps < 48) { ++c1; ++steps; }
-        if (steps >= 48) { seek(Q, -1); return; }
-        set(c1);
-    }
-};
-
-// First query q in [qa, qb] with Qg(q) >= x (qb if none) on a non-decreasing
-// query grid: an inverse-linspace estimate fixed up by exact comparisons, or a
-// bisection when the grid is a table.
-__device__ __forceinline__ int first_q_ge(const Grid1 &Qg, double inv_step, double x, int qa, int qb) {
-    int q;
-    if (!Qg.tab) {
-        const double e = ceil(__dmul_rn(__dsub_rn(x, Qg.start), inv_step));
-        q = (e == e) ? (int)fmin(fmax(e, (double)qa), (double)qb) : qa;
-        int steps = 0;
-        while (q > qa && Qg(q - 1) >= x && steps < 16) { --q; ++steps; }
-        while (q < qb && Qg(q) < x && steps < 32) { ++q; ++steps; }
-        if (steps < 16) return q;
-    }
-    int lo = qa, hi = qb;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (Qg(mid) < x) lo = mid + 1; else hi = mid;
-    }
-    return lo;
-}
-
-// Scattered bracket of one output: J (ascending-view source index) and the
-// sinc phase packed as (J << 8) | ph; BR_NONE = not reached by the scatter,
-// BR_ZERO = outside the sources (np.interp's left/right = 0), BR_LAST = the
-// query equals the last source position (value of the last sample).
-constexpr int BR_NONE = -1, BR_ZERO = -2, BR_LAST = -3;
-
-// Queries of [qa, qb) below the first source x_first or above the last x_last.
-__device__ __forceinline__ void scatter_outside(const Grid1 &Qg, double inv_step, bool first,
-                                                bool last, double x_first, double x_last, int qa,
-                                                int qb, int *brk, int qbase) {
-    if (first) {
-        const int qs = first_q_ge(Qg, inv_step, x_first, qa, qb);
-        for (int q = qa; q < qs; ++q) brk[q - qbase] = BR_ZERO;
-    }
-    if (last) {
-        for (int q = first_q_ge(Qg, inv_step, x_last, qa, qb); q < qb; ++q)
-            brk[q - qbase] = (Qg(q) == x_last) ? BR_LAST : BR_ZERO;
-    }
+    return make_float2(ar, ai);
 }
 
 // Sum a per-thread count over the wave and add it once (fallback statistics).
@@ -272,195 +135,213 @@ __device__ __forceinline__ void count_wave(unsigned *dst, unsigned v) {
     if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, v);
 }
 
-// Bracket scatter for the source interval [x0, x1) of ascending source u:
-// every query of [qa, qb) inside it gets (u, phase) and, when frq is given,
-// its fraction (Q - x0) / (x1 - x0).
-__device__ __forceinline__ void scatter_interval(const Grid1 &Qg, double inv_step, int u, double x0,
-                                                 double x1, int qa, int qb, int *brk, float *frq,
-                                                 int qbase) {
-    if (!(x0 < x1)) return;                                   // empty interval
-    const int qs = first_q_ge(Qg, inv_step, x0, qa, qb);
-    const double dx = __dsub_rn(x1, x0);
-    for (int q = qs; q < qb; ++q) {
-        const double Q = Qg(q);
-        if (!(Q < x1)) break;
-        const double frac = __ddiv_rn(__dsub_rn(Q, x0), dx);
-        const int wq = (int)fmin(fmax(rint(frac * 32768.0), 0.0), 32767.0);
-        brk[q - qbase] = (u << 8) | (wq >> 7);
-        if (frq) frq[q - qbase] = (float)frac;
-    }
-}
-
-// Exact per-output value of the azimuth pass (row edges, scatter misses): a
-// bisection over the sources KR_j * tan_s and np.interp / SINC32 there.  Kept
-// out of line so the unrolled gather stays small.
-template <bool SINC>
-__device__ __noinline__ float2 azimuth_slow(double krj, const double *__restrict__ tan_s, int m,
-                                            double Q, const float2 *row, const float *tapl) {
-    auto xa = [&](int u) { return __dmul_rn(krj, tan_s[u]); };
-    auto fa = [&](int u) { return row[u]; };
-    Walk<decltype(xa)> w(xa, m);
-    w.seek(Q, -1);
-    const Bracket bb = classify(SINC, w.c, m, Q, w.xprev, w.xcur, xa(m - 1));
-    return value_slow(bb, Q, fa, tapl);
-}
-
 // ------------------------------------------------------------------ pass 1 (range)
 constexpr int RP = 16;                   // pulses per tile
-constexpr int RQ = 128;                  // range bins per tile (8 per thread)
-constexpr int RS = 176;                  // staged samples per pulse (RQ + 37 + slack)
-constexpr int RSP = 196;                 // padded strip row: spad(RS-1) < RSP and RSP = 4 mod 32,
-                                         // so a wave's 4 pulses start 8 banks apart
+constexpr int RQ = 256;                  // range bins per tile
+constexpr int RT = 512;                  // threads: 16 pulses x 32 groups of 4 bins, twice
+constexpr int RS = 304;                  // staged samples per pulse (RQ + window + slack)
+constexpr int RSP = RS + 1;              // odd pitch: a wave's 16 pulses hit different banks
+constexpr int RLD = (RP * RS + RT - 1) / RT;   // strip samples loaded per thread
+
+// Exact count c = #{u : xa(u) <= Q} over an ascending view of S positions,
+// walking from a guess (a few steps), bisecting when the guess is far off;
+// leaves xl = xa(c - 1), xh = xa(c) (+-inf past the ends).
+template <typename XA>
+__device__ __forceinline__ int bracket_from(XA &xa, int S, double Q, int g, double &xl, double &xh) {
+    int c = min(max(g, 0), S);
+    xl = c > 0 ? xa(c - 1) : -INFINITY;
+    xh = c < S ? xa(c) : INFINITY;
+    if ((xl <= Q) && !(xh <= Q)) return c;
+    int steps = 0;
+    while (c > 0 && !(xl <= Q) && steps < 8) { --c; ++steps; xh = xl; xl = c > 0 ? xa(c - 1) : -INFINITY; }
+    while (c < S && xh <= Q && steps < 8) { ++c; ++steps; xl = xh; xh = c < S ? xa(c) : INFINITY; }
+    if ((c == 0 || xl <= Q) && (c == S || !(xh <= Q))) return c;
+    int lo = 0, hi = S;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (xa(mid) <= Q) lo = mid + 1; else hi = mid;
+    }
+    c = lo;
+    xl = c > 0 ? xa(c - 1) : -INFINITY;
+    xh = c < S ? xa(c) : INFINITY;
+    return c;
+}
 
 // Persistent: each workgroup stages the tap table once, then walks tiles of
-// RP sorted pulses x RQ range bins.  Per tile: (A) one FP64 position per source
-// sample, each source interval scattering its bracket to the range bins inside
-// it; (B) a branch-light gather over those brackets.  Outputs the scatter did
-// not reach (edges, or a strip estimate that fell short) take the per-output
-// walk.
+// RP sorted pulses x RQ range bins (its pulse groups in turn, each swept
+// along range so strip halos are L2 hits).  Per tile: every pulse's source
+// strip goes to LDS while each thread finds the FP64 brackets of its 8 bins
+// (closed-form guess from the uniform frequency grid, exact comparisons);
+// then four-bin shared-window sinc sums (or np.interp) and a transpose
+// through LDS so the corner-turned g1t rows are written 128 bytes at a time.
+// Lane layout: thread t serves pulse t % 16 and bin groups t / 16 and
+// t / 16 + 32, so the lanes of a wave read 16 different strips.
 template <bool SINC, bool FULLFREQ>
-__global__ __launch_bounds__(256, 2) void k_range_f32(
+__global__ __launch_bounds__(RT, 2) void k_range_f32(
     const float2 *__restrict__ sig, int m, int n, const int32_t *__restrict__ order,
     const double *__restrict__ a64, const double *__restrict__ f0v, const double *__restrict__ dfv,
     const double *__restrict__ freq, const Grid1 KR,
     const int16_t *__restrict__ sinc, float2 *__restrict__ g1t, int ldp,
     unsigned *__restrict__ nslow) {
-    static_assert(sizeof(float2) * RQ * (RP + 1) <= sizeof(float2) * RP * RSP, "tile fits the strip");
-    __shared__ float2 strip[RP][RSP];                  // source strips, then the output tile
-    __shared__ int brk[RP * RQ];
-    __shared__ float frq[SINC ? 1 : RP * RQ];
+    static_assert(sizeof(float2) * RQ * (RP + 1) <= sizeof(float2) * RP * RSP, "tile fits the strips");
+    static_assert(RQ == 4 * 2 * (RT / RP), "two groups of four bins per thread");
+    __shared__ float2 strip[RP * RSP];                 // source strips, then the output tile
     __shared__ float tapl[SINC ? TROWS * 256 : 1];
-    __shared__ int lo_s[RP], hi_s[RP];
-    auto tile = reinterpret_cast<float2 (*)[RP + 1]>(&strip[0][0]);
-    unsigned slow = 0;
-    const int t = threadIdx.x;
-    const int pl = t >> 4, ch = t & 15;
+    __shared__ int lo_s[RP], hi_s[RP], st_s[RP], dir_s[RP];
+    auto tile = reinterpret_cast<float2 (*)[RP + 1]>(&strip[0]);
     if (SINC) load_taps(tapl, sinc);
-    const double inv_step = KR.tab ? 0.0 : 1.0 / KR.step;
     const int tiles_p = (m + RP - 1) / RP, tiles_q = (n + RQ - 1) / RQ;
-    // each workgroup owns whole pulse groups and sweeps their range tiles in
-    // order: strip halos of consecutive tiles are L2 hits, and concurrently
-    // running workgroups write different pulse columns of the same g1t rows
     const int my_groups = blockIdx.x < tiles_p ? (tiles_p - 1 - blockIdx.x) / gridDim.x + 1 : 0;
+    unsigned slow = 0;
     for (int it = 0; it < my_groups * tiles_q; ++it) {
         const int p0 = (blockIdx.x + (it / tiles_q) * gridDim.x) * RP, q0 = (it % tiles_q) * RQ;
         const int qe = min(q0 + RQ, n);
+        int t = threadIdx.x;                          // opaque: per-thread values stay in the loop
+        asm volatile("" : "+v"(t));
+        const int pl = t % RP, gi = t / RP;           // pulse in tile, bin group
         const int p = p0 + pl;
-        const int qa = q0 + ch * 8;                   // this thread's first output
-        __syncthreads();                              // previous tile fully consumed
-        for (int r = t; r < RP * RQ; r += 256) brk[r] = BR_NONE;
+        // per-pulse position model kr(j) = (2 pr / c) f(j), f(j) = f0 + j df
         int i = 0;
         double ai = 1.0, f0 = 0.0, df = 1.0;
-        bool asc = true;
         if (p < m) {
             i = order[p];
             ai = a64[i];
-            if (!FULLFREQ) { f0 = f0v[i]; df = dfv[i]; }
+            if (FULLFREQ) {
+                f0 = freq[(size_t)i * n];
+                df = __ddiv_rn(__dsub_rn(freq[(size_t)i * n + n - 1], f0), (double)(n - 1));
+            } else {
+                f0 = f0v[i];
+                df = dfv[i];
+            }
         }
-        const double *frow = FULLFREQ ? freq + (size_t)i * n : nullptr;
-        // kr = (2 f / c) pr as (2 pr / c) * f: within 2 ulp of numpy's rounding
+        const double *frow = freq + (size_t)i * n;
         auto kr = [&](int j) {
             const double f = FULLFREQ ? frow[j] : __dadd_rn(f0, __dmul_rn((double)j, df));
             return __dmul_rn(ai, f);
         };
-        if (p < m) asc = !(kr(0) > kr(n - 1));
+        const bool asc = p < m ? !(kr(0) > kr(n - 1)) : true;
         auto xa = [&](int u) { return kr(asc ? u : n - 1 - u); };
+        const double x0 = ai * f0, inv_adf = 1.0 / (ai * df);
+        auto guess = [&](double Q) {                  // ascending-view count estimate
+            const double js = (Q - x0) * inv_adf;
+            const double g = asc ? js : (double)(n - 1) - js;
+            return (g == g) ? (int)fmin(fmax(floor(g) + 1.0, 0.0), (double)n) : 0;
+        };
         const float2 *row = sig + (size_t)i * n;
         auto fa_g = [&](int u) { return row[asc ? u : n - 1 - u]; };
-        auto guess = [&](double Q) {                  // ascending-view count estimate
-            if (FULLFREQ) return -1;
-            const double js = (Q / ai - f0) / df;
-            const double g = asc ? js : (double)(n - 1) - js;
-            if (!(g == g)) return -1;
-            return (int)fmin(fmax(floor(g) + 1.0, 0.0), (double)n);
-        };
-        if (ch == 0 && p < m) {                       // strip: first bracket minus taps ..
-            const double Q = KR(min(q0, n - 1));
-            Walk<decltype(xa)> w(xa, n);
-            w.seek(Q, guess(Q));
-            const int lo = max(0, min(w.c - 1 - 15 - 2, n - RS));
-            int len = RS;                             // .. to the estimated last bracket
-            const int gl = guess(KR(qe - 1));
-            if (gl >= 0) len = min(RS, max(0, gl + 16 + 3 - lo));
-            lo_s[pl] = lo;
-            hi_s[pl] = min(lo + len, n);
+        __syncthreads();                              // previous tile fully consumed
+        if (t < RP) {                                 // pulse t's strip: first window on, RS samples
+            int lo = 0, hi = 0;
+            if (p < m) {
+                double xl, xh;
+                const double Q = KR(q0);
+                const int c = bracket_from(xa, n, Q, guess(Q), xl, xh);
+                lo = max(0, min(c - 1 - 15 - 2, n - RS));
+                hi = min(lo + RS, n);
+            }
+            lo_s[t] = lo;
+            hi_s[t] = hi;
+            st_s[t] = asc ? lo : n - 1 - lo;          // sample index of strip slot 0
+            dir_s[t] = asc ? 1 : -1;
         }
         __syncthreads();
-        const int lo = lo_s[pl], hi = hi_s[pl];       // strip holds [lo, hi)
-        if (p < m) {
-            for (int u = ch; u < hi - lo; u += 16)
-                strip[pl][spad(u)] = ld_stream(row + (asc ? lo + u : n - 1 - lo - u));
-            // (A) scatter: source intervals [x_u, x_u+1) of the strip
-            for (int u = lo + ch; u < hi - 1; u += 16)
-                scatter_interval(KR, inv_step, u, xa(u), xa(u + 1), q0, qe, brk + pl * RQ,
-                                 SINC ? nullptr : frq + pl * RQ, q0);
-            if (ch == 0 && (lo == 0 || hi == n))
-                scatter_outside(KR, inv_step, lo == 0, hi == n, xa(0), xa(n - 1), q0, qe,
-                                brk + pl * RQ, q0);
+        // strip loads in flight while the brackets are found
+        float2 pre[RLD];
+#pragma unroll
+        for (int k = 0; k < RLD; ++k) {
+            const int r = t + k * RT;
+            const int pp = r / RS, u = r - (r / RS) * RS;
+            pre[k] = make_float2(0.f, 0.f);
+            if (pp < RP && lo_s[pp] + u < hi_s[pp])
+                pre[k] = ld_stream(sig + (size_t)order[p0 + pp] * n + st_s[pp] + dir_s[pp] * u);
+        }
+        // brackets of this thread's bins: (c << 8) | phase and the fraction
+        int code[8];
+        float frq[8];
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int k = g * 4 + u;
+                const int q = q0 + 4 * (gi + g * (RT / RP)) + u;
+                code[k] = 0;
+                frq[k] = 0.f;
+                if (p >= m || q >= qe) continue;
+                const double Q = KR(q);
+                double xl, xh;
+                const int c = bracket_from(xa, n, Q, guess(Q), xl, xh);
+                int ph = 0;
+                if (c == n) {
+                    ph = (Q == xl) ? 1 : 0;           // last sample, or outside
+                } else if (c > 0) {
+                    const double fr = __ddiv_rn(__dsub_rn(Q, xl), __dsub_rn(xh, xl));
+                    ph = (int)fmin(rint(fr * 32768.0), 32767.0) >> 7;
+                    frq[k] = (float)fr;
+                }
+                code[k] = (c << 8) | ph;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < RLD; ++k) {
+            const int r = t + k * RT;
+            const int pp = r / RS, u = r - (r / RS) * RS;
+            if (pp < RP) strip[pp * RSP + u] = pre[k];
         }
         __syncthreads();
-        float2 vals[8];
-        if (p < m) {
-            auto fa = [&](int u) { return (u >= lo && u < hi) ? strip[pl][spad(u - lo)] : fa_g(u); };
-            // (B) gather, four outputs per shared window
+        // gather
+        float2 v[8];
+        const int lo = lo_s[pl], hi = hi_s[pl];
+        const float2 *sp = strip + pl * RSP;
+        auto fa = [&](int u) { return (u >= lo && u < hi) ? sp[u - lo] : fa_g(u); };
 #pragma unroll
-            for (int g = 0; g < 2; ++g) {
-                int e[4], d[4], ph[4];
-                bool fast = SINC;
+        for (int g = 0; g < 2; ++g) {
+            int d[4], ph[4];
+            const int J0 = (code[g * 4] >> 8) - 1;
+            bool fast = SINC && p < m;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int J = (code[g * 4 + u] >> 8) - 1;
+                d[u] = J - J0;
+                ph[u] = code[g * 4 + u] & 255;
+                fast = fast && J >= 15 && J <= n - 17 && d[u] >= 0 && d[u] <= KD &&
+                       q0 + 4 * (gi + g * (RT / RP)) + u < qe;
+            }
+            fast = fast && J0 - 15 >= lo && J0 - 15 + KW <= hi;
+            if (fast) {
+                float2 w4[4];
+                sinc4u(sp, J0 - 15 - lo, tapl, ph, d, w4);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) v[g * 4 + u] = w4[u];
+            } else {
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const int q = qa + g * 4 + u;
-                    e[u] = q < qe ? brk[pl * RQ + q - q0] : BR_NONE;
-                }
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int J = e[u] >> 8;
-                    d[u] = J - (e[0] >> 8);
-                    ph[u] = e[u] & 255;
-                    fast = fast && e[u] != BR_NONE && J >= 15 && J <= n - 17 && d[u] >= 0 && d[u] <= KD;
-                }
-                const int s0 = (e[0] >> 8) - 15;
-                fast = fast && s0 >= lo && s0 + KW <= hi;
-                float2 v[4];
-                if (fast) {
-                    sinc4(strip[pl], s0 - lo, tapl, ph, d, v);
-                } else {
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int q = qa + g * 4 + u;
-                        const int J = e[u] >> 8;
-                        if (q >= qe || e[u] == BR_ZERO) {
-                            v[u] = make_float2(0.f, 0.f);
-                        } else if (e[u] == BR_LAST) {
-                            v[u] = fa(n - 1);
-                        } else if (!SINC && e[u] != BR_NONE) {        // linear from the scatter
-                            const float fr = frq[pl * RQ + q - q0];
-                            const float2 a0 = fa(J), a1 = fa(J + 1);
-                            v[u] = make_float2(fmaf(a1.x - a0.x, fr, a0.x), fmaf(a1.y - a0.y, fr, a0.y));
-                        } else if (e[u] >= 0) {                       // sinc off the shared window
-                            v[u] = value_scattered<SINC>(e[u], n, KR(q), xa(J), xa(J + 1), fa, tapl);
-                        } else {                                      // exact per-output path
+                    const int k = g * 4 + u;
+                    const int q = q0 + 4 * (gi + g * (RT / RP)) + u;
+                    const int c = code[k] >> 8;
+                    float2 r = make_float2(0.f, 0.f);
+                    if (p < m && q < qe && c > 0) {
+                        if (c == n) {
+                            if (code[k] & 255) r = fa(n - 1);
+                        } else if (SINC && c - 1 >= 15 && c - 1 <= n - 17) {
                             ++slow;
-                            const double Q = KR(q);
-                            Walk<decltype(xa)> w(xa, n);
-                            w.seek(Q, guess(Q));
-                            const Bracket bb = classify(SINC, w.c, n, Q, w.xprev, w.xcur, xa(n - 1));
-                            v[u] = value_slow(bb, Q, fa, tapl);
+                            r = sinc1(fa, c - 1, code[k] & 255, tapl);
+                        } else {
+                            const float2 a0 = fa(c - 1), a1 = fa(c);
+                            const float fr = frq[k];
+                            r = make_float2(fmaf(a1.x - a0.x, fr, a0.x), fmaf(a1.y - a0.y, fr, a0.y));
                         }
                     }
+                    v[k] = r;
                 }
-#pragma unroll
-                for (int u = 0; u < 4; ++u) vals[g * 4 + u] = v[u];
             }
         }
         __syncthreads();                              // strips dead: the tile takes over
-        if (p < m) {
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
-                if (qa + u < qe) tile[qa + u - q0][pl] = vals[u];
-        }
+        for (int g = 0; g < 2; ++g)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) tile[4 * (gi + g * (RT / RP)) + u][pl] = v[g * 4 + u];
         __syncthreads();
-        for (int r = t; r < RQ * RP; r += 256) {
+        for (int r = t; r < RQ * RP; r += RT) {
             const int qq = r / RP, pp = r % RP;
             if (q0 + qq < n && p0 + pp < m) st_stream(g1t + (size_t)(q0 + qq) * ldp + p0 + pp, tile[qq][pp]);
         }
@@ -479,25 +360,38 @@ __device__ __forceinline__ int xcd_map(int b, int count) {
     return (b % 8) * per + b / 8;
 }
 
-// Elements per thread of the azimuth kernel's FFT: 16 (512 threads) for the
-// 8192-point line so all waves stay busy, 32 below.
+// Elements per thread of the azimuth kernel's FFT and resample: 8, so an
+// 8192-point line runs on 1024 threads (16 waves keep the CU busy while one
+// workgroup holds the LDS).
 template <int L>
-constexpr int az_emax() { return 32; }
+constexpr int az_emax() { return 8; }
 template <int L>
 constexpr int az_nb() { return pfa32::Cfg<L, az_emax<L>()>::NB; }
 
+// Bracket-guess table over tan(phi): idx[b] = #{p : tan_s[p] < lo + b / inv_w}.
+struct TanBins {
+    const int32_t *idx;
+    double lo, inv_w;
+    int bins;
+    __device__ __forceinline__ int guess(double t) const {
+        const double b = floor(__dmul_rn(__dsub_rn(t, lo), inv_w));
+        return idx[(b == b) ? (int)fmin(fmax(b, 0.0), (double)bins) : 0];
+    }
+};
+
 // Persistent over range bins: iteration it of workgroup b (XCD x = b % 8, rank
-// b / 8) takes j = x * ceil(n/8) + it * (grid/8) + rank.
-// Per range bin: stage the sorted-pulse row, (A) scatter each source interval
-// [KR_j tan_s[p], KR_j tan_s[p+1]) to the KC queries inside it, (B) gather
-// four outputs per shared window, window and sign-fold into the LDS line, FFT.
+// b / 8) takes j = x * ceil(n/8) + it * (grid/8) + rank.  One LDS region holds,
+// in turn, the line's FP64 source positions KR_j tan_s[p] (bracket search), the
+// sorted-pulse row (the gather's source, prefetched into registers during the
+// previous line) and the windowed, sign-folded line the FFT runs on.
+// Thread t owns outputs 4 (g NB + t) + u, u < 4: the four share one sinc window.
 // MODE 0: store the windowed k-space g2[q][j] (resample_kspace stage output)
 // MODE 1: fused -- window, sign fold, pad, FFT along cross -> A (row-major)
 // MODE 2: FFT along cross of a given g2 (focus() stage) -> A (row-major)
 template <int L, int MODE, bool SINC, bool INV>
 __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
     const float2 *__restrict__ g1t, int ldp, int m, int n, const Grid1 KR,
-    const double *__restrict__ tan_s, const Grid1 KC,
+    const double *__restrict__ tan_s, const TanBins tb, const Grid1 KC,
     const double *__restrict__ hm, const double *__restrict__ hn,
     const int16_t *__restrict__ sinc, const float2 *__restrict__ g2in,
     float2 *__restrict__ out, const float2 *__restrict__ twg, unsigned *__restrict__ nslow) {
@@ -505,107 +399,181 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
     using Cf = pfa32::Cfg<L, EM>;
     constexpr int M2 = Cf::N;
     constexpr int NB = Cf::NB;
+    constexpr int G = (M2 + 4 * NB - 1) / (4 * NB);   // 4-output groups per thread
+    constexpr int PR = (M2 + NB - 1) / NB;             // row samples per thread
     constexpr bool RES = MODE != 2;                    // resampling modes
-    __shared__ float2 buf[Cf::PADN];                   // the FFT line
+    __shared__ float2 reg[Cf::PADN + 64];              // positions | row | FFT line
     __shared__ float tapl[(SINC && RES) ? TROWS * 256 : 1];
     __shared__ float2 twl[M2 / 64 + 64];
-    __shared__ int brk[RES ? M2 : 1];
-    __shared__ float frq[(RES && !SINC) ? M2 : 1];
+    double *xs = reinterpret_cast<double *>(reg);
     const int t = threadIdx.x;
     if (SINC && RES) load_taps(tapl, sinc);
     const pfa32::Tw tw = pfa32::load_tw<M2>(twl, twg);
-    const double inv_step = KC.tab ? 0.0 : 1.0 / KC.step;
     const int per_x = (n + 7) / 8;
     const int ranks = (gridDim.x + 7) / 8;
     const int x = blockIdx.x % 8, rank = blockIdx.x / 8;
+    auto line_of = [&](int base) { return x * per_x + base + rank; };
+    auto active_at = [&](int base) { return base + rank < per_x && line_of(base) < n; };
     unsigned slow = 0;
+    float2 pre[PR];                                    // next row, in flight
+    auto prefetch = [&](int j) {
+        const float2 *row = g1t + (size_t)j * ldp;
+#pragma unroll
+        for (int k = 0; k < PR; ++k) {
+            const int p = t + k * NB;
+            pre[k] = p < m ? ld_stream(row + p) : make_float2(0.f, 0.f);
+        }
+    };
+    if (RES && active_at(0)) prefetch(line_of(0));
     for (int base = 0; base < per_x; base += ranks) {
-        const int jl = base + rank;
-        const int j = x * per_x + jl;
-        const bool active = jl < per_x && j < n;     // uniform per workgroup
-        if (!active) continue;
-        __syncthreads();                              // previous line fully consumed
+        if (!active_at(base)) continue;               // uniform per workgroup
+        const int j = line_of(base);
+        // per-output values (KC(q), tapers) are recomputed per line: hoisted
+        // out of the loop they would pin ~40 registers for its whole length
+        int tl = t;
+        asm volatile("" : "+v"(tl));
+        float2 v[G * 4];
         if (MODE == 2) {
+            __syncthreads();                          // previous line fully consumed
             for (int q = t; q < M2; q += NB) {
-                float2 v = q < m ? g2in[(size_t)q * n + j] : make_float2(0.f, 0.f);
-                if ((q + j) & 1) v = make_float2(-v.x, -v.y);
-                buf[pfa32::padf(q)] = v;
+                float2 w = q < m ? g2in[(size_t)q * n + j] : make_float2(0.f, 0.f);
+                if ((q + j) & 1) w = make_float2(-w.x, -w.y);
+                reg[pfa32::padf(q)] = w;
             }
         } else {
-            // the sorted-pulse row is read through L1: each window sample feeds
-            // four outputs, and outputs go straight into the FFT line
-            const float2 *row = g1t + (size_t)j * ldp;
-            for (int u = t; u < m; u += NB) brk[u] = BR_NONE;
-            __syncthreads();
             const double krj = KR(j);
-            auto xa = [&](int u) { return __dmul_rn(krj, tan_s[u]); };
-            auto fa = [&](int u) { return row[u]; };
-            // (A) scatter the source intervals to the KC queries
-            for (int pp = t; pp < m - 1; pp += NB)
-                scatter_interval(KC, inv_step, pp, xa(pp), xa(pp + 1), 0, m, brk,
-                                 SINC ? nullptr : frq, 0);
-            if (t == 0) scatter_outside(KC, inv_step, true, false, xa(0), 0.0, 0, m, brk, 0);
-            if (t == NB - 1) scatter_outside(KC, inv_step, false, true, 0.0, xa(m - 1), 0, m, brk, 0);
+            const double inv_krj = 1.0 / krj;
+            // (1) FP64 source positions of this line
+            __syncthreads();                          // previous line fully consumed
+            for (int p = t; p < m; p += NB) xs[p] = __dmul_rn(krj, tan_s[p]);
             __syncthreads();
-            const double hj = hn ? hn[j] : 1.0;
-            // (B) gather: sub-group g of thread t covers outputs 4 (g NB + t) .. +3,
-            // so a wave's lanes read neighbouring windows
-            for (int g = 0; g * NB * 4 < M2; ++g) {
-                const int qa = (g * NB + t) * 4;
-                int e[4], d[4], ph[4];
+            // (2) brackets: c = #{p : xs[p] <= Q} (np.interp / searchsorted-right),
+            // kept as (c << 8) | sinc phase, and the interval fraction
+            int code[G * 4];
+            float frq[G * 4];
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int q = (g * NB + tl) * 4 + u;
+                    const int k = g * 4 + u;
+                    code[k] = 0;
+                    frq[k] = 0.f;
+                    if (q >= m) continue;
+                    const double Q = KC(q);
+                    int c = tb.guess(__dmul_rn(Q, inv_krj));
+                    double xl = c > 0 ? xs[c - 1] : -INFINITY;
+                    double xh = c < m ? xs[c] : INFINITY;
+                    if (xh <= Q) {                    // the guess is low by one for ~1/4 of outputs
+                        ++c;
+                        xl = xh;
+                        xh = c < m ? xs[c] : INFINITY;
+                    }
+                    if (!(xl <= Q) || xh <= Q) {
+                        ++slow;
+                        while (c > 0 && !(xl <= Q)) { --c; xh = xl; xl = c > 0 ? xs[c - 1] : -INFINITY; }
+                        while (c < m && xh <= Q) { ++c; xl = xh; xh = c < m ? xs[c] : INFINITY; }
+                    }
+                    int ph = 0;
+                    if (c == m) {
+                        ph = (Q == xl) ? 1 : 0;                              // last sample, or outside
+                    } else if (c > 0) {
+                        const double fr = __ddiv_rn(__dsub_rn(Q, xl), __dsub_rn(xh, xl));
+                        ph = (int)fmin(rint(fr * 32768.0), 32767.0) >> 7;
+                        frq[k] = (float)fr;
+                    }
+                    code[k] = (c << 8) | ph;
+                }
+            }
+            __syncthreads();                          // positions dead: the row takes over
+            // (3) the row (unpadded), zero past m so shifted windows read zeros
+#pragma unroll
+            for (int k = 0; k < PR; ++k) {
+                const int p = t + k * NB;
+                if (p < M2) reg[p] = pre[k];
+            }
+            if (t < 48) reg[m + t] = make_float2(0.f, 0.f);
+            __syncthreads();
+            // (4) gather
+            auto fa = [&](int u) { return reg[u]; };
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                int d[4], ph[4];
                 bool fast = SINC;
+                const int J0 = (code[g * 4] >> 8) - 1;
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const int q = qa + u;
-                    e[u] = q < m ? brk[q] : BR_NONE;
+                    const int J = (code[g * 4 + u] >> 8) - 1;
+                    d[u] = J - J0;
+                    ph[u] = code[g * 4 + u] & 255;
+                    fast = fast && J >= 15 && J <= m - 17 && d[u] >= 0 && d[u] <= KD;
                 }
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int J = e[u] >> 8;
-                    d[u] = J - (e[0] >> 8);
-                    ph[u] = e[u] & 255;
-                    fast = fast && e[u] != BR_NONE && J >= 15 && J <= m - 17 && d[u] >= 0 && d[u] <= KD;
-                }
-                const int s0 = (e[0] >> 8) - 15;
-                fast = fast && s0 + KW <= m;            // the whole window lies in the row
-                float2 v[4];
                 if (fast) {
-                    sinc4<false>(row, s0, tapl, ph, d, v);
+                    float2 w4[4];
+                    sinc4u(reg, J0 - 15, tapl, ph, d, w4);
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) v[g * 4 + u] = w4[u];
                 } else {
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
-                        const int q = qa + u;
-                        const int J = e[u] >> 8;
-                        if (q >= m || e[u] == BR_ZERO) {
-                            v[u] = make_float2(0.f, 0.f);
-                        } else if (e[u] == BR_LAST) {
-                            v[u] = fa(m - 1);
-                        } else if (!SINC && e[u] != BR_NONE) {        // linear from the scatter
-                            const float fr = frq[q];
-                            const float2 a0 = fa(J), a1 = fa(J + 1);
-                            v[u] = make_float2(fmaf(a1.x - a0.x, fr, a0.x), fmaf(a1.y - a0.y, fr, a0.y));
-                        } else if (e[u] >= 0) {                       // sinc off the shared window
-                            v[u] = value_scattered<SINC>(e[u], m, KC(q), xa(J), xa(J + 1), fa, tapl);
-                        } else {                                      // exact per-output path
-                            ++slow;
-                            v[u] = azimuth_slow<SINC>(krj, tan_s, m, KC(q), row, tapl);
+                        const int k = g * 4 + u;
+                        const int q = (g * NB + tl) * 4 + u;
+                        const int c = code[k] >> 8;
+                        float2 r = make_float2(0.f, 0.f);
+                        if (q < m && c > 0) {
+                            if (c == m) {
+                                if (ph[u]) r = fa(m - 1);
+                            } else if (SINC && c - 1 >= 15 && c - 1 <= m - 17) {
+                                r = sinc1(fa, c - 1, ph[u], tapl);
+                            } else {
+                                const float2 f0 = fa(c - 1), f1 = fa(c);
+                                const float fr = frq[k];
+                                r = make_float2(fmaf(f1.x - f0.x, fr, f0.x), fmaf(f1.y - f0.y, fr, f0.y));
+                            }
                         }
+                        v[k] = r;
                     }
                 }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            {                                         // the next row, in flight during the FFT
+                const int nb2 = base + ranks;
+                if (nb2 < per_x && active_at(nb2)) prefetch(line_of(nb2));
+            }
+            // window
+            const double hj = hn ? hn[j] : 1.0;
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const int q = qa + u;
-                    if (q >= M2) break;
-                    float2 xv = v[u];
+                    const int q = (g * NB + tl) * 4 + u;
                     if (q < m && hm) {
                         const float wgt = (float)__dmul_rn(hm[q], hj);
-                        xv = make_float2(xv.x * wgt, xv.y * wgt);
+                        v[g * 4 + u] = make_float2(v[g * 4 + u].x * wgt, v[g * 4 + u].y * wgt);
                     }
-                    if (MODE == 0) {
-                        if (q < m) out[(size_t)q * n + j] = xv;
-                    } else {
-                        if ((q + j) & 1) xv = make_float2(-xv.x, -xv.y);
-                        buf[pfa32::padf(q)] = xv;
+                }
+            }
+            if (MODE == 0) {
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int q = (g * NB + tl) * 4 + u;
+                        if (q < m) out[(size_t)q * n + j] = v[g * 4 + u];
+                    }
+                }
+                continue;
+            }
+            __syncthreads();                          // the row is dead: the FFT line takes over
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int q = (g * NB + tl) * 4 + u;
+                    if (q < M2) {
+                        float2 w = v[g * 4 + u];
+                        if ((q + j) & 1) w = make_float2(-w.x, -w.y);
+                        reg[pfa32::padf(q)] = w;
                     }
                 }
             }
@@ -614,9 +582,9 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
             __syncthreads();
             float2 *dst = out + ablk(j, 0, M2);
             const float scale = INV ? 1.0f / (float)M2 : 1.0f;
-            pfa32::fft<L, INV, EM>(buf, tw, [&](int idx, float2 v) {
-                if (INV) v = make_float2(v.x * scale, v.y * scale);
-                st_stream(dst + idx, v);
+            pfa32::fft<L, INV, EM>(reg, tw, [&](int idx, float2 w) {
+                if (INV) w = make_float2(w.x * scale, w.y * scale);
+                st_stream(dst + idx, w);
             });
         }
     }
@@ -696,19 +664,17 @@ constexpr int nb_of() { return pfa32::Cfg<L>::NT < 64 ? 64 : pfa32::Cfg<L>::NT; 
 
 int launch_float_range(pfa_plan *p, const void *sig, const double *freq, void *g1t, int ldp) {
     pfa_ctx *ctx = p->ctx;
-    const int tiles = ((p->m + RP - 1) / RP) * ((p->n + RQ - 1) / RQ);
     const int tiles_p = (p->m + RP - 1) / RP;
     dim3 grid(std::max(1, std::min(tiles_p, ctx->num_cu * 2)));
-    (void)tiles;
     const bool sinc = p->flags & PFA_FLOAT_SINC32;
     const bool full = freq != nullptr;
     pfa_timer_begin(ctx, PFA_TIMER_RANGE);
 #define ARGS (const float2 *)sig, p->m, p->n, p->order, p->a64, p->f0d, p->dfd, freq, grid_of(p->KR, p->kr_lin, p->n), p->sinc, \
              (float2 *)g1t, ldp, (unsigned *)(ctx->d_counters + 32)
-    if (sinc && full) hipLaunchKernelGGL((k_range_f32<true, true>), grid, dim3(256), 0, ctx->stream, ARGS);
-    else if (sinc) hipLaunchKernelGGL((k_range_f32<true, false>), grid, dim3(256), 0, ctx->stream, ARGS);
-    else if (full) hipLaunchKernelGGL((k_range_f32<false, true>), grid, dim3(256), 0, ctx->stream, ARGS);
-    else hipLaunchKernelGGL((k_range_f32<false, false>), grid, dim3(256), 0, ctx->stream, ARGS);
+    if (sinc && full) hipLaunchKernelGGL((k_range_f32<true, true>), grid, dim3(RT), 0, ctx->stream, ARGS);
+    else if (sinc) hipLaunchKernelGGL((k_range_f32<true, false>), grid, dim3(RT), 0, ctx->stream, ARGS);
+    else if (full) hipLaunchKernelGGL((k_range_f32<false, true>), grid, dim3(RT), 0, ctx->stream, ARGS);
+    else hipLaunchKernelGGL((k_range_f32<false, false>), grid, dim3(RT), 0, ctx->stream, ARGS);
 #undef ARGS
     pfa_timer_end(ctx, PFA_TIMER_RANGE);
     PFA_HIP(ctx, hipGetLastError());
@@ -728,7 +694,8 @@ int launch_float_azimuth(pfa_plan *p, int mode, const void *g1t, int ldp, const 
     // persistent: one resident workgroup per CU, a multiple of 8 (one per XCD slot)
     const int grid_az = std::max(8, std::min((p->n + 7) / 8 * 8, ctx->num_cu / 8 * 8));
     pfa_timer_begin(ctx, PFA_TIMER_AZIMUTH);
-#define ARGS (const float2 *)g1t, ldp, p->m, p->n, grid_of(p->KR, p->kr_lin, p->n), p->tan_s64, grid_of(p->KC, p->kc_lin, p->m), p->hm, p->hn, p->sinc, \
+    const TanBins tb{p->tan_bin, p->tb_lo, p->tb_inv_w, p->tb_bins};
+#define ARGS (const float2 *)g1t, ldp, p->m, p->n, grid_of(p->KR, p->kr_lin, p->n), p->tan_s64, tb, grid_of(p->KC, p->kc_lin, p->m), p->hm, p->hn, p->sinc, \
              (const float2 *)g2in, (float2 *)out, tw, (unsigned *)(ctx->d_counters + 32)
 #define K4(LL, MODE)                                                                                 \
     if (sinc && inv) hipLaunchKernelGGL((k_azimuth_f32<LL, MODE, true, true>), dim3(grid_az), dim3(az_nb<LL>()), 0, ctx->stream, ARGS); \

@@ -54,6 +54,11 @@ struct pfa_plan {
     // exactly such grids (the kernels then form them in registers)
     double kr_lin_v[3] = {0, 0, 0}, kc_lin_v[3] = {0, 0, 0};
     const double *kr_lin = nullptr, *kc_lin = nullptr;
+    // bracket guesses of the azimuth pass: tan_bin[b] = #{p : tan_s[p] < tb_lo + b / tb_inv_w},
+    // b = 0..tb_bins (a starting point only; the kernels fix brackets up exactly)
+    int32_t *tan_bin = nullptr;
+    int tb_bins = 0;
+    double tb_lo = 0.0, tb_inv_w = 0.0;
     std::vector<void *> owned;            // device allocations of the plan
 };
 
