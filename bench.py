@@ -126,7 +126,7 @@ def time_fixed(sc, ax, ay, geom, steps, warmup, torch):
     return wall, kern
 
 
-def cpu_baseline(spec, interp, geom, budget_rows=160):
+def cpu_baseline(spec, interp, geom, budget_rows=400):
     """The numpy oracle on the host, on a bounded sample of the frame: pass 1 on
     `budget_rows` pulses and pass 2 on `budget_rows` range bins of the real
     geometry (each scaled to the full frame), plus the full 2-D FFT + detect."""
@@ -238,13 +238,30 @@ def main():
         "kernels_gbps": {k: round(bytes_k[k] / (kern[k] * 1e-3) / 1e9, 1) for k in kern},
         "fallback_outputs_per_frame": [round(c / (a.steps + a.warmup)) for c in fb],
     }
+    # end to end through the host-buffer entry point (pageable numpy in and
+    # out, as the reference's ctypes caller would pass them): never `value`
+    sig_host = sc.sig.cpu().numpy()
+    plan.focus(sig_host, detect=True)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        plan.focus(sig_host, detect=True)
+    res["end_to_end_host_buffers_ms"] = round((time.perf_counter() - t0) / 3 * 1e3, 2)
+    del sig_host
     del out
     plan.close()
     if a.copy_calibration:
+        # plain 512 MiB device copies of known size for the HBM byte counters:
+        # torch's elementwise copy and the HIP runtime's copy kernel
+        import ctypes
         x = torch.empty(512 << 20, dtype=torch.uint8, device="cuda")
         y = torch.empty_like(x)
+        hip = ctypes.CDLL("libamdhip64.so")
+        hip.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                       ctypes.c_int, ctypes.c_void_p]
         for _ in range(3):
             y.copy_(x)
+            hip.hipMemcpyAsync(y.data_ptr(), x.data_ptr(), 512 << 20, 3,
+                               torch.cuda.current_stream().cuda_stream)
         torch.cuda.synchronize()
         res["copy_calibration_bytes"] = 512 << 20
         del x, y

@@ -66,9 +66,10 @@ int pfa_timer_reset(pfa_ctx *ctx);
 enum { PFA_TIMER_RANGE = 0, PFA_TIMER_AZIMUTH = 1, PFA_TIMER_FFT_B = 2,
        PFA_TIMER_DETECT = 3, PFA_TIMER_OTHER = 4, PFA_TIMER_KINDS = 5 };
 
-/* Counters of outputs that took the exact per-output fallback instead of the
- * scattered-bracket fast path since the last reset: [0] range pass, [1]
- * azimuth pass (edges of the sample lines land there by design). */
+/* Diagnostic counters since the last reset: [0] range-pass sinc outputs whose
+ * 32-tap window left the staged strip (read from global memory instead), [1]
+ * azimuth-pass outputs whose bracket needed the exact fix-up walk beyond the
+ * table guess. */
 int pfa_fallback_counts(pfa_ctx *ctx, uint32_t out[2], int reset);
 
 /* ------------------------------------------------------------------ tables */
