@@ -57,11 +57,12 @@ def make_inputs(spec, device):
     return sc, ax, ay, geom
 
 
-def algorithmic_bytes_float(m, n, m2, n2, s_in=8, s_mid=8, s_out=4):
-    """Compulsory HBM bytes per kernel of the 3-kernel float staging."""
+def algorithmic_bytes_float(m, n, m2, n2, s_in=8, s_mid=8, s_a=8, s_out=4):
+    """Compulsory HBM bytes per kernel of the 3-kernel float staging (s_a: bytes
+    per element of A, 4 in the matrix-core forms that store it as 16-bit pairs)."""
     return {"range": s_in * m * n + s_mid * n * m,          # read phase history, write g1t
-            "azimuth_fft": s_mid * n * m + s_mid * n * m2,  # read g1t, write A
-            "fft_b_detect": s_mid * n * m2 + s_out * m2 * n2}
+            "azimuth_fft": s_mid * n * m + s_a * n * m2,    # read g1t, write A
+            "fft_b_detect": s_a * n * m2 + s_out * m2 * n2}
 
 
 def time_float(plan, sig, out, steps, warmup, torch):
@@ -126,6 +127,55 @@ def time_fixed(sc, ax, ay, geom, steps, warmup, torch):
     return wall, kern
 
 
+def precision_forms(spec, sc, geom, interp, forms, steps, warmup, torch):
+    """Time each FFT form on the frame and state its error against an FP64
+    reference: the centred 2-D FFT in double precision (on the GPU) of this
+    frame's FP32 k-space (the resample stage every form shares)."""
+    from mi355x_pfa import metrics
+    from mi355x_pfa.pfa import FloatPlan
+    m, n = spec.pulses, spec.samples
+    base = FloatPlan(m, n, -1, geom, f0=sc.sc0, df=sc.scss, window=True, interp=interp)
+    m2, n2 = base.m2, base.n2
+    g2 = base.resample_kspace(sc.sig.cpu().numpy())
+    base.close()
+    pad = torch.zeros((m2, n2), dtype=torch.complex128, device="cuda")
+    pad[:m, :n] = torch.from_numpy(g2).to("cuda")
+    del g2
+    ref = torch.fft.fftshift(torch.fft.fft2(torch.fft.ifftshift(pad)))
+    del pad
+    ref_abs = ref.abs()
+    peak = float(ref_abs.max())
+    ref_host = ref_abs.float().cpu().numpy()
+    out = {}
+    for form in forms:
+        plan = FloatPlan(m, n, -1, geom, f0=sc.sc0, df=sc.scss, window=True, interp=interp,
+                         fft=form)
+        det = torch.empty((m2, n2), dtype=torch.float32, device="cuda")
+        wall, kern = time_float(plan, sc.sig, det, steps, warmup, torch)
+        img = torch.empty((m2, n2), dtype=torch.complex64, device="cuda")
+        plan.focus_dev(sc.sig, img, detect=False)
+        torch.cuda.synchronize()
+        err = float(torch.sqrt(torch.mean((img.to(torch.complex128) - ref).abs() ** 2)) / peak)
+        mag = img.abs().cpu().numpy()
+        pk = metrics.top_peaks(ref_host, 1)[0]
+        ps, isl = metrics.pslr_islr(mag, pk)
+        ps_r, isl_r = metrics.pslr_islr(ref_host, pk)
+        bk = algorithmic_bytes_float(m, n, m2, n2, s_a=8 if form == "fp32" else 4)
+        out[form] = {"ms_per_frame": round(wall, 4),
+                     "kernels_ms": {k: round(v, 4) for k, v in kern.items()},
+                     "kernels_gbps": {k: round(bk[k] / (kern[k] * 1e-3) / 1e9, 1) for k in kern},
+                     "error_vs_fp64": {"pn_rms": float(f"{err:.3e}"),
+                                       "peak_shift_px": metrics.peak_shift(mag, ref_host, 8),
+                                       "pslr_db": round(ps, 3), "islr_db": round(isl, 3),
+                                       "pslr_fp64_db": round(ps_r, 3),
+                                       "islr_fp64_db": round(isl_r, 3)}}
+        del det, img, mag
+        plan.close()
+    del ref, ref_abs
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(spec, interp, geom, budget_rows=400):
     """The numpy oracle on the host, on a bounded sample of the frame: pass 1 on
     `budget_rows` pulses and pass 2 on `budget_rows` range bins of the real
@@ -179,6 +229,10 @@ def main():
                     help="16 targets, no clutter (same shapes; for profiler runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 512 MiB device copy (counter calibration)")
+    ap.add_argument("--fft", default="fp32", choices=["fp32", "bf16", "fp16", "bf16-split"],
+                    help="FFT form of the timed frames (value)")
+    ap.add_argument("--forms", default="fp32,bf16,fp16",
+                    help="FFT forms timed beside, with their error against FP64 ('' = none)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: the focus path is built and timed on a single "
@@ -200,13 +254,14 @@ def main():
     sc, ax, ay, geom = make_inputs(spec, "cuda")
     ctx = context(0)
     ctx.set_stream(torch.cuda.current_stream().cuda_stream)
-    plan = FloatPlan(m, n, -1, geom, f0=sc.sc0, df=sc.scss, window=True, interp=a.interp)
+    plan = FloatPlan(m, n, -1, geom, f0=sc.sc0, df=sc.scss, window=True, interp=a.interp,
+                     fft=a.fft)
     out = torch.empty((plan.m2, plan.n2), dtype=torch.float32, device="cuda")
     print("[bench] float frames", file=sys.stderr, flush=True)
     ctx.fallback_counts(reset=True)
     wall, kern = time_float(plan, sc.sig, out, a.steps, a.warmup, torch)
     fb = ctx.fallback_counts(reset=True)
-    bytes_k = algorithmic_bytes_float(m, n, plan.m2, plan.n2)
+    bytes_k = algorithmic_bytes_float(m, n, plan.m2, plan.n2, s_a=8 if a.fft == "fp32" else 4)
     frame_dev = sum(kern.values())
     B3 = sum(bytes_k.values())
     dom = max(kern, key=kern.get)
@@ -222,7 +277,8 @@ def main():
         "metric": METRIC, "value": round(wall, 4), "unit": "ms/frame",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(wall, 4),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic (GPU-synthesised spotlight phase history)",
+        "dtype": "f32" if a.fft == "fp32" else f"f32 resample, {a.fft} matrix-core FFT",
+        "data": "synthetic (GPU-synthesised spotlight phase history)",
         "config": {"workload": f"{a.config}: {m} pulses x {n} samples -> "
                                f"{plan.m2}x{plan.n2} PFA frame, {a.interp} both passes, FP32",
                    "pulses": m, "samples": n, "grid": plan.m2, "interp": a.interp,
@@ -265,6 +321,11 @@ def main():
         torch.cuda.synchronize()
         res["copy_calibration_bytes"] = 512 << 20
         del x, y
+    forms = [f for f in a.forms.split(",") if f]
+    if forms:
+        print("[bench] FFT forms vs FP64", file=sys.stderr, flush=True)
+        res["fft_forms"] = precision_forms(spec, sc, geom, a.interp, forms, a.steps, a.warmup,
+                                           torch)
     if not a.no_fixed and a.config == "C3":
         print("[bench] fixed-point frames", file=sys.stderr, flush=True)
         fwall, fk = time_fixed(sc, ax, ay, geom, max(3, a.steps // 2), 2, torch)

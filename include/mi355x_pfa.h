@@ -158,6 +158,15 @@ int pfa_form_image_fixed_dev(pfa_plan *plan, const int16_t *sig_iq, uint16_t *ou
  *   sgn            <0: fft2 (SGN=-1), >0: ifft2 (transform2d :144-147)
  * The image is the zero-padded (to_pow2) centred transform, [cross][range]. */
 #define PFA_FLOAT_SINC32 0x1        /* 32-tap polyphase gather both passes     */
+/* Precision form of pfa_focus_f32[_dev] (no reference counterpart; the FP32
+ * form above is the reference's arithmetic): both FFT passes as dense
+ * DFT-matrix stages on the matrix cores with 16-bit operands and FP32
+ * accumulation, per-line power-of-two block scaling, the intermediate stored
+ * as 16-bit complex pairs.  Padded sizes m2, n2 in {256, 512, 1024, 4096,
+ * 8192}.  pfa_transform_f32 and pfa_resample_kspace_f32 stay FP32. */
+#define PFA_FLOAT_FFT_BF16 0x2      /* BF16 operands                           */
+#define PFA_FLOAT_FFT_FP16 0x4      /* FP16 operands                           */
+#define PFA_FLOAT_FFT_SPLIT 0x8     /* with BF16: DFT matrices as hi + lo parts */
 enum { PFA_OUT_C64 = 0, PFA_OUT_ABS_F32 = 1 };
 int pfa_plan_float(pfa_ctx *ctx, pfa_plan **plan, int m, int n, int sgn,
                    const double *pr, const int32_t *order, const double *tan_s,

@@ -508,7 +508,12 @@ extern "C" int pfa_plan_float(pfa_ctx *ctx, pfa_plan **out, int m, int n, int sg
         return pfa_fail(ctx, PFA_ERR_ARG, "plan_float: null table");
     if ((f0 == nullptr) != (df == nullptr)) return pfa_fail(ctx, PFA_ERR_ARG, "plan_float: give both f0 and df or neither");
     if ((ham_c == nullptr) != (ham_r == nullptr)) return pfa_fail(ctx, PFA_ERR_ARG, "plan_float: give both tapers or neither");
-    if (flags & ~PFA_FLOAT_SINC32) return pfa_fail(ctx, PFA_ERR_ARG, "plan_float: unknown flag");
+    if (flags & ~(PFA_FLOAT_SINC32 | PFA_FLOAT_FFT_BF16 | PFA_FLOAT_FFT_FP16 | PFA_FLOAT_FFT_SPLIT))
+        return pfa_fail(ctx, PFA_ERR_ARG, "plan_float: unknown flag");
+    if ((flags & PFA_FLOAT_FFT_BF16) && (flags & PFA_FLOAT_FFT_FP16))
+        return pfa_fail(ctx, PFA_ERR_ARG, "plan_float: choose one of FFT_BF16 and FFT_FP16");
+    if ((flags & PFA_FLOAT_FFT_SPLIT) && !(flags & PFA_FLOAT_FFT_BF16))
+        return pfa_fail(ctx, PFA_ERR_ARG, "plan_float: FFT_SPLIT applies to FFT_BF16");
     if ((flags & PFA_FLOAT_SINC32) && !sinc_tab) return pfa_fail(ctx, PFA_ERR_ARG, "plan_float: SINC32 needs the tap table");
     if (sgn == 0) return pfa_fail(ctx, PFA_ERR_ARG, "plan_float: sgn must be -1 or +1");
     if (m < 2 || n < 2) return pfa_fail(ctx, PFA_ERR_DIMS, "plan_float: need m, n >= 2");
@@ -516,6 +521,11 @@ extern "C" int pfa_plan_float(pfa_ctx *ctx, pfa_plan **out, int m, int n, int sg
     if (m2 > 8192 || n2 > 8192) return pfa_fail(ctx, PFA_ERR_DIMS, "plan_float: padded grid above 8192");
     if (pow2_ceil(m) != m2 || pow2_ceil(n) != n2)
         return pfa_fail(ctx, PFA_ERR_DIMS, "plan_float: m and n must be at least 33 (padded size >= 64)");
+    if (flags & (PFA_FLOAT_FFT_BF16 | PFA_FLOAT_FFT_FP16)) {
+        auto ok_size = [](int v) { return v == 256 || v == 512 || v == 1024 || v == 4096 || v == 8192; };
+        if (!ok_size(m2) || !ok_size(n2))
+            return pfa_fail(ctx, PFA_ERR_DIMS, "plan_float: matrix-core FFT sizes are 256, 512, 1024, 4096 and 8192");
+    }
     std::vector<int> seen(m, 0);
     for (int i = 0; i < m; ++i) {
         if (order[i] < 0 || order[i] >= m || seen[order[i]]++) return pfa_fail(ctx, PFA_ERR_GEOMETRY, "order is not a permutation");
@@ -606,6 +616,18 @@ extern "C" int pfa_focus_f32_dev(pfa_plan *p, const float *sig, const double *fr
     const int ldp = ldp32(p->m);
     void *g1t = pfa_ws(ctx, "f_g1t", (size_t)p->n * ldp * 8, &st);
     if (!g1t) return st;
+    const int fk = (p->flags & PFA_FLOAT_FFT_BF16) ? ((p->flags & PFA_FLOAT_FFT_SPLIT) ? 3 : 1)
+                   : (p->flags & PFA_FLOAT_FFT_FP16) ? 2 : 0;
+    if (fk) {                                  // precision form: A as 16-bit pairs + row exponents
+        unsigned *A16 = (unsigned *)pfa_ws(ctx, "f_A16", (size_t)p->n * p->m2 * 4, &st);
+        if (!A16) return st;
+        int *expA = (int *)pfa_ws(ctx, "f_expA", (size_t)p->n * 4, &st);
+        if (!expA) return st;
+        CK(launch_float_range(p, sig, freq, g1t, ldp));
+        CK(launch_float_azimuth_mfma(p, g1t, ldp, A16, expA, fk));
+        CK(launch_float_fft_b_mfma(p, A16, expA, out, out_kind, fk));
+        return PFA_OK;
+    }
     void *A = pfa_ws(ctx, "f_A", (size_t)(p->n + 15) / 16 * 16 * p->m2 * 8, &st);
     if (!A) return st;
     CK(launch_float_range(p, sig, freq, g1t, ldp));

@@ -1,0 +1,335 @@
+// pfa_fft_mfma.hpp -- N-point FFT (N = 2^L, L in {8, 9, 10, 12, 13}) as dense
+// DFT-matrix stages on the gfx950 matrix cores, 16-bit operands (BF16 or FP16)
+// with FP32 accumulation.
+//
+// A radix-R stage (R = 16 or 32) of a Stockham FFT is, for each group of 16
+// vectors, the real GEMM  [Yr; Yi] = M [Xr; Xi]  with the 2R x 2R real
+// embedding of the R-point DFT,
+//     forward  M = [[C, S], [-S, C]],   inverse  M = [[C, -S], [S, C]],
+//     C[m][r] = cos(2 pi m r / R),  S[m][r] = sin(2 pi m r / R),
+// run as v_mfma_f32_16x16x32_{bf16,f16}: A = 16 rows of M (registers, built
+// once per thread), B = 32 K-values of 16 vectors (one 16-byte LDS read per
+// lane), C = 16 x 16 FP32 outputs.  Lane layouts (checked on the device):
+//     A[l % 16][8 (l / 16) + i],  B[8 (l / 16) + i][l % 16],  C[4 (l / 16) + i][l % 16].
+//
+// Data between stages lives in LDS as two 16-bit planes (real, imaginary),
+// each stored vector-major for the stage that reads it: element e = k + r N/R
+// of a radix-R stage sits at k R + r, padded by 8 slots every 64 so the
+// 16-byte operand reads of 16 lanes (32 or 64 bytes apart) use distinct banks.
+// The twiddle of the next stage is applied in FP32 to the accumulators before
+// they are rounded and stored.
+//
+// Block scaling: every stage multiplies by 1/R (exact), so the transform is
+// DFT/N; the caller normalises each line to max |component| <= 1 with a
+// power-of-two exponent s and restores 2^(s + L) (forward) or 2^s (inverse,
+// numpy's ifft already divides by N).  Magnitudes stay below 2^1.5 inside.
+// SPLIT: the DFT matrix as hi + lo 16-bit parts (two products per tile), so
+// its rounding error drops from 2^-9 (BF16) to below 2^-16.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "pfa_fft_f32.hpp"
+
+namespace pfam {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+template <bool BF>
+struct H16 {
+    using T = __bf16;
+    using V = bf16x8;
+};
+template <>
+struct H16<false> {
+    using T = _Float16;
+    using V = f16x8;
+};
+
+__device__ __forceinline__ f32x4 mma(const bf16x8 &a, const bf16x8 &b, const f32x4 &c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f32x4 mma(const f16x8 &a, const f16x8 &b, const f32x4 &c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+
+// 16-bit storage of a float (round to nearest even) and back.
+template <bool BF>
+__device__ __forceinline__ unsigned short to16(float x) {
+    const typename H16<BF>::T h = (typename H16<BF>::T)x;
+    return __builtin_bit_cast(unsigned short, h);
+}
+template <bool BF>
+__device__ __forceinline__ float from16(unsigned short b) {
+    return (float)__builtin_bit_cast(typename H16<BF>::T, b);
+}
+
+// Radix plan: stage radices (first stage first).
+template <int L>
+struct MPlan;
+template <> struct MPlan<8>  { static constexpr int NS = 2; static constexpr int R[3] = {16, 16, 1}; };
+template <> struct MPlan<9>  { static constexpr int NS = 2; static constexpr int R[3] = {32, 16, 1}; };
+template <> struct MPlan<10> { static constexpr int NS = 2; static constexpr int R[3] = {32, 32, 1}; };
+template <> struct MPlan<12> { static constexpr int NS = 3; static constexpr int R[3] = {16, 16, 16}; };
+template <> struct MPlan<13> { static constexpr int NS = 3; static constexpr int R[3] = {32, 16, 16}; };
+
+template <int L>
+constexpr bool supported() { return L == 8 || L == 9 || L == 10 || L == 12 || L == 13; }
+
+// Waves that share NCOL lines: at most the block's, at most the 16-vector
+// groups of the stage with the fewest.
+template <int L, int NCOL, int BLOCK_WAVES>
+constexpr int nw_for() {
+    int g = 1 << 30;
+    for (int s = 0; s < MPlan<L>::NS; ++s) {
+        const int gs = NCOL * ((1 << L) / MPlan<L>::R[s]) / 16;
+        g = gs < g ? gs : g;
+    }
+    return BLOCK_WAVES < g ? BLOCK_WAVES : g;
+}
+
+template <int L, int S>
+constexpr int ls_of() {                      // product of the radices before stage S
+    int p = 1;
+    for (int s = 0; s < S; ++s) p *= MPlan<L>::R[s];
+    return p;
+}
+
+// Padded slot of position p in one 16-bit plane; plane length incl. padding.
+__device__ __forceinline__ int pp(int p) { return p + ((p >> 6) << 3); }
+template <int L>
+constexpr int plane_len() { return (1 << L) + ((1 << L) >> 3); }
+
+// Position of element e in the plane layout read by stage S.
+template <int L, int S>
+__device__ __forceinline__ int pos_of(int e) {
+    constexpr int R = MPlan<L>::R[S], NV = (1 << L) / R;
+    return (e % NV) * R + e / NV;
+}
+
+// cos / sin of 2 pi t / 32 for t in [0, 32), from the 16-entry table of the FP32 FFT.
+__device__ __forceinline__ float cos32(int t) {
+    t &= 31;
+    return t < 16 ? pfa32::kC32[t][0] : -pfa32::kC32[t - 16][0];
+}
+__device__ __forceinline__ float sin32(int t) {
+    t &= 31;
+    return t < 16 ? pfa32::kC32[t][1] : -pfa32::kC32[t - 16][1];
+}
+
+// A operand of tile (mt, ks) of the radix-R real DFT matrix for this lane:
+// hi part and (SPLIT) the lo remainder.
+template <int R, bool BF, bool INV>
+__device__ __forceinline__ void dft_operand(int mt, int ks, int l, typename H16<BF>::V &hi,
+                                            typename H16<BF>::V &lo) {
+    const int row = 16 * mt + (l & 15);
+    const int br = row / R, mm = row % R;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int col = 32 * ks + 8 * (l >> 4) + i;
+        const int bc = col / R, rr = col % R;
+        const int t = (mm * rr) % R * (32 / R);
+        const float c = cos32(t), s = sin32(t);
+        float v;
+        if (br == bc) v = c;
+        else if (br == 0) v = INV ? -s : s;            // (0, 1)
+        else v = INV ? s : -s;                         // (1, 0)
+        const typename H16<BF>::T h = (typename H16<BF>::T)v;
+        hi[i] = h;
+        lo[i] = (typename H16<BF>::T)(v - (float)h);
+    }
+}
+
+// Tiles (mt, ks) of the DFT matrices of all stages, stage S's first tile.
+template <int L>
+constexpr int ntiles() {
+    int t = 0;
+    for (int s = 0; s < MPlan<L>::NS; ++s) t += (2 * MPlan<L>::R[s] / 16) * (2 * MPlan<L>::R[s] / 32);
+    return t;
+}
+template <int L, int S>
+constexpr int tile0() {
+    int t = 0;
+    for (int s = 0; s < S; ++s) t += (2 * MPlan<L>::R[s] / 16) * (2 * MPlan<L>::R[s] / 32);
+    return t;
+}
+
+// DFT-matrix operands computed in registers for each stage call.
+template <int L, bool BF, bool INV>
+struct OpsReg {
+    template <int S>
+    __device__ __forceinline__ void get(int mt, int ks, int lane, typename H16<BF>::V &hi,
+                                        typename H16<BF>::V &lo) const {
+        dft_operand<MPlan<L>::R[S], BF, INV>(mt, ks, lane, hi, lo);
+    }
+};
+
+// DFT-matrix operands from a table built once per workgroup (build_ops):
+// tab[t * 64 + lane] = hi of tile t, tab[(ntiles + t) * 64 + lane] = lo.
+template <int L, bool BF, bool INV>
+struct OpsLds {
+    const u16x8 *tab;
+    template <int S>
+    __device__ __forceinline__ void get(int mt, int ks, int lane, typename H16<BF>::V &hi,
+                                        typename H16<BF>::V &lo) const {
+        constexpr int KS = 2 * MPlan<L>::R[S] / 32;
+        const int t = tile0<L, S>() + mt * KS + ks;
+        hi = __builtin_bit_cast(typename H16<BF>::V, tab[t * 64 + lane]);
+        lo = __builtin_bit_cast(typename H16<BF>::V, tab[(ntiles<L>() + t) * 64 + lane]);
+    }
+};
+
+template <int L, bool BF, bool INV, int S>
+__device__ __forceinline__ void build_ops_stage(u16x8 *tab) {
+    if constexpr (S < MPlan<L>::NS) {
+        constexpr int R = MPlan<L>::R[S], MT = 2 * R / 16, KS = 2 * R / 32;
+        for (int i = threadIdx.x; i < MT * KS * 64; i += blockDim.x) {
+            const int tt = i / 64, lane = i % 64;
+            typename H16<BF>::V hi, lo;
+            dft_operand<R, BF, INV>(tt / KS, tt % KS, lane, hi, lo);
+            tab[(tile0<L, S>() + tt) * 64 + lane] = __builtin_bit_cast(u16x8, hi);
+            tab[(ntiles<L>() + tile0<L, S>() + tt) * 64 + lane] = __builtin_bit_cast(u16x8, lo);
+        }
+        build_ops_stage<L, BF, INV, S + 1>(tab);
+    }
+}
+template <int L, bool BF, bool INV>
+__device__ __forceinline__ OpsLds<L, BF, INV> build_ops(u16x8 *tab) {
+    build_ops_stage<L, BF, INV, 0>(tab);
+    return OpsLds<L, BF, INV>{tab};
+}
+
+// One stage over NCOL lines held in `planes` (per line: re plane then im
+// plane, plane_len<L>() slots each).  Stage S reads its layout, applies the
+// DFT, scales by 1/R, then either (not last) multiplies by the next stage's
+// twiddle and writes the next layout, or (last) hands (line, index, value) to
+// sink.  NW waves share the work; results wait in registers for the barrier.
+template <int L, bool BF, bool INV, bool SPLIT, int NCOL, int NW, int S, typename Ops, typename Sink>
+__device__ __forceinline__ void stage(unsigned short *planes, const pfa32::Tw &tw, const Ops &ops,
+                                      Sink &sink) {
+    constexpr int N = 1 << L;
+    constexpr int R = MPlan<L>::R[S];
+    constexpr int NV = N / R;
+    constexpr int LS = ls_of<L, S>();
+    constexpr bool LAST = S == MPlan<L>::NS - 1;
+    constexpr int MT = 2 * R / 16, KS = 2 * R / 32;
+    constexpr int GROUPS = NCOL * NV / 16;             // 16-vector groups over all lines
+    static_assert(NW >= 1 && GROUPS % NW == 0, "groups split evenly over the waves");
+    constexpr int GPW = GROUPS / NW;
+    constexpr int PL = plane_len<L>();
+    using V = typename H16<BF>::V;
+    int lane = threadIdx.x & 63;
+    asm volatile("" : "+v"(lane));
+    const int w = threadIdx.x >> 6;
+    const int n = lane & 15, q = lane >> 4;
+    V Ah[MT][KS], Al[MT][KS];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) ops.template get<S>(mt, ks, lane, Ah[mt][ks], Al[mt][ks]);
+    f32x4 acc[GPW][MT];
+    const bool active = w < NW;                        // waves beyond NW only keep the barriers
+#pragma unroll
+    for (int g = 0; g < GPW; ++g) {
+        if (!active) break;
+        const int G = w + g * NW;                      // global group
+        const int col = G / (NV / 16), k = (G % (NV / 16)) * 16 + n;
+        const unsigned short *pl = planes + col * 2 * PL;
+        V b[KS];
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            const int kc = 32 * ks + 8 * q;            // first K row of this lane
+            const int plane = kc / R, r0 = kc % R;
+            const u16x8 raw = *reinterpret_cast<const u16x8 *>(pl + plane * PL + pp(k * R + r0));
+            b[ks] = __builtin_bit_cast(V, raw);
+        }
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            f32x4 c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                c = mma(Ah[mt][ks], b[ks], c);
+                if (SPLIT) c = mma(Al[mt][ks], b[ks], c);
+            }
+            acc[g][mt] = c;
+        }
+    }
+    __syncthreads();                                   // every operand read before any write
+    constexpr float inv_r = 1.0f / (float)R;
+#pragma unroll
+    for (int g = 0; g < GPW; ++g) {
+        if (!active) break;
+        const int G = w + g * NW;
+        const int col = G / (NV / 16), k = (G % (NV / 16)) * 16 + n;
+        const int j = k % LS;
+        const int obase = (k / LS) * LS * R + j;
+        unsigned short *pl = planes + col * 2 * PL;
+#pragma unroll
+        for (int mt = 0; mt < MT / 2; ++mt) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int m = 16 * mt + 4 * q + i;     // output radix index
+                const int o = obase + m * LS;
+                float2 y = make_float2(acc[g][mt][i] * inv_r, acc[g][mt + MT / 2][i] * inv_r);
+                if constexpr (LAST) {
+                    sink(col, o, y);
+                } else {
+                    constexpr int R2 = MPlan<L>::R[S + 1];
+                    constexpr int NV2 = N / R2;
+                    constexpr int LS2 = LS * R;
+                    const int k2 = o % NV2, r2 = o / NV2;
+                    const int j2 = k2 % LS2;
+                    if (j2 * r2) {
+                        float2 wv = tw(j2 * r2 * (N / (LS2 * R2)));
+                        if (INV) wv.y = -wv.y;
+                        y = pfa32::cmul(y, wv);
+                    }
+                    const int p = pp(k2 * R2 + r2);
+                    pl[p] = to16<BF>(y.x);
+                    pl[PL + p] = to16<BF>(y.y);
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+template <int L, bool BF, bool INV, bool SPLIT, int NCOL, int NW, int S, typename Ops, typename Sink>
+__device__ __forceinline__ void stages_from(unsigned short *planes, const pfa32::Tw &tw,
+                                            const Ops &ops, Sink &sink) {
+    if constexpr (S < MPlan<L>::NS) {
+        stage<L, BF, INV, SPLIT, NCOL, NW, S>(planes, tw, ops, sink);
+        stages_from<L, BF, INV, SPLIT, NCOL, NW, S + 1>(planes, tw, ops, sink);
+    }
+}
+
+// Store element e of line col (first-stage layout), already scaled.
+template <int L, bool BF>
+__device__ __forceinline__ void put(unsigned short *planes, int col, int e, float2 v) {
+    constexpr int PL = plane_len<L>();
+    const int p = pp(pos_of<L, 0>(e));
+    planes[col * 2 * PL + p] = to16<BF>(v.x);
+    planes[col * 2 * PL + PL + p] = to16<BF>(v.y);
+}
+
+// Transform NCOL lines whose first-stage layout is in `planes` (after a
+// barrier); sink(col, index, value) receives DFT/N (or IDFT/N) in FP32.
+template <int L, bool BF, bool INV, bool SPLIT, int NCOL, int NW, typename Ops, typename Sink>
+__device__ __forceinline__ void fft(unsigned short *planes, const pfa32::Tw &tw, const Ops &ops,
+                                    Sink &&sink) {
+    static_assert(supported<L>(), "MFMA FFT sizes: 256, 512, 1024, 4096, 8192");
+    stages_from<L, BF, INV, SPLIT, NCOL, NW, 0>(planes, tw, ops, sink);
+}
+
+// Power-of-two exponent s with max_component * 2^-s <= 1 (0 for an all-zero line).
+__device__ __forceinline__ int norm_exp(float mx) {
+    if (!(mx > 0.f)) return 0;
+    int e;
+    frexpf(mx, &e);                                    // mx = f 2^e, f in [0.5, 1)
+    return e;
+}
+
+}  // namespace pfam

@@ -1,0 +1,698 @@
+// pfa_float_dev.hpp -- device code of the float (FP32) Polar Format focus,
+// shared by the FP32 launchers (pfa_float.hip) and the matrix-core FFT
+// launchers (pfa_mfma.hip).  See pfa_float.hip for the kernel list.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <vector>
+
+#include "pfa_fft_f32.hpp"
+#include "pfa_fft_mfma.hpp"
+#include "pfa_internal.hpp"
+
+namespace {
+
+constexpr double C_LIGHT = 299792458.0;
+
+// ------------------------------------------------------------------ shared machinery
+// A uniform grid as np.linspace forms it -- g[k] = k * step + start (each op
+// rounded), g[cnt-1] = stop -- or an explicit table when the grid is not one.
+struct Grid1 {
+    const double *tab;
+    double start, step, stop;
+    int cnt;
+    __device__ __forceinline__ double operator()(int k) const {
+        if (tab) return tab[k];
+        return k == cnt - 1 ? stop : __dadd_rn(__dmul_rn((double)k, step), start);
+    }
+};
+
+typedef float f2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 ld_stream(const float2 *p) {      // read-once data
+    const f2v v = __builtin_nontemporal_load(reinterpret_cast<const f2v *>(p));
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void st_stream(float2 *p, float2 v) {     // write-once data
+    f2v w;
+    w.x = v.x;
+    w.y = v.y;
+    __builtin_nontemporal_store(w, reinterpret_cast<f2v *>(p));
+}
+// Taps of the Q15 sinc table as float (exact: c / 2^15), staged in LDS
+// tap-major: tap t of phase ph at tapl[(t + 4) * 256 + scr(ph)], with four zero
+// rows on either side (t in [-4, 36)) so a window shifted by up to KD samples
+// reads zeros instead of needing masks, and scr(ph) = ph ^ (ph >> 3) spreads
+// the phases of a wave's lanes over the LDS banks.
+constexpr int TROWS = 40;
+__device__ __forceinline__ int scr(int ph) { return ph ^ (ph >> 3); }
+__device__ __forceinline__ void load_taps(float *tapl, const int16_t *__restrict__ sinc) {
+    for (int i = threadIdx.x; i < TROWS * 256; i += blockDim.x) {
+        const int row = i >> 8, ph = i & 255, t = row - 4;
+        tapl[row * 256 + scr(ph)] = (t >= 0 && t < 32) ? (float)sinc[ph * 32 + t] * (1.0f / 32768.0f) : 0.f;
+    }
+}
+
+constexpr int KD = 4;            // bracket advance covered by one 4-output window
+constexpr int KW = 32 + KD;      // samples in that window
+
+// Four consecutive outputs whose brackets J_u = J_0 + d_u (0 <= d_u <= KD) share
+// one window w[k] = row[J_0 - 15 + k], k < KW; output u uses taps shifted by
+// d_u: sum_k tap[ph_u][k - d_u] w[k].  Each window sample is read once per four
+// outputs instead of once per output.
+// The row is an unpadded LDS array: every window read is an LDS load with an
+// immediate offset, so the loop is pure loads and FMAs.
+template <int BATCH = 6>
+__device__ __forceinline__ void sinc4u(const float2 *row, int start, const float *tapl,
+                                       const int (&ph)[4], const int (&d)[4], float2 (&out)[4]) {
+    const float *col[4];
+    float ar[4], ai[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        col[u] = tapl + (4 - d[u]) * 256 + scr(ph[u]);
+        ar[u] = 0.f;
+        ai[u] = 0.f;
+    }
+    const float2 *ws = row + start;
+    static_assert(KW % BATCH == 0, "window splits into batches");
+    // a rolled loop over batches: fully unrolled, the compiler front-loads all
+    // 180 window and tap loads and runs out of registers
+#pragma unroll 1
+    for (int kb = 0; kb < KW; kb += BATCH) {
+#pragma unroll
+        for (int k = 0; k < BATCH; ++k) {
+            const float2 w = ws[k];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const float c = col[u][k * 256];
+                ar[u] = fmaf(c, w.x, ar[u]);
+                ai[u] = fmaf(c, w.y, ai[u]);
+            }
+        }
+        ws += BATCH;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) col[u] += BATCH * 256;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) out[u] = make_float2(ar[u], ai[u]);
+}
+
+// SINC32 value of one interior output (bracket J, phase ph) read through fa.
+template <typename FA>
+__device__ __forceinline__ float2 sinc1(FA &&fa, int J, int ph, const float *tapl) {
+    const float *col = tapl + 4 * 256 + scr(ph);
+    float ar = 0.f, ai = 0.f;
+#pragma unroll 8
+    for (int t = 0; t < 32; ++t) {
+        const float2 s = fa(J - 15 + t);
+        ar = fmaf(col[t * 256], s.x, ar);
+        ai = fmaf(col[t * 256], s.y, ai);
+    }
+    return make_float2(ar, ai);
+}
+
+// Sum a per-thread count over the wave and add it once (fallback statistics).
+__device__ __forceinline__ void count_wave(unsigned *dst, unsigned v) {
+    if (!dst) return;
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, v);
+}
+
+// ------------------------------------------------------------------ pass 1 (range)
+constexpr int RP = 16;                   // pulses per tile
+constexpr int RQ = 256;                  // range bins per tile
+constexpr int RT = 512;                  // threads: 16 pulses x 32 groups of 4 bins, twice
+constexpr int RS = 304;                  // staged samples per pulse (RQ + window + slack)
+constexpr int RSP = RS + 1;              // odd pitch: a wave's 16 pulses hit different banks
+constexpr int RLD = (RP * RS + RT - 1) / RT;   // strip samples loaded per thread
+
+// Exact count c = #{u : xa(u) <= Q} over an ascending view of S positions,
+// walking from a guess (a few steps), bisecting when the guess is far off;
+// leaves xl = xa(c - 1), xh = xa(c) (+-inf past the ends).
+template <typename XA>
+__device__ __forceinline__ int bracket_from(XA &xa, int S, double Q, int g, double &xl, double &xh) {
+    int c = min(max(g, 0), S);
+    xl = c > 0 ? xa(c - 1) : -INFINITY;
+    xh = c < S ? xa(c) : INFINITY;
+    if ((xl <= Q) && !(xh <= Q)) return c;
+    int steps = 0;
+    while (c > 0 && !(xl <= Q) && steps < 8) { --c; ++steps; xh = xl; xl = c > 0 ? xa(c - 1) : -INFINITY; }
+    while (c < S && xh <= Q && steps < 8) { ++c; ++steps; xl = xh; xh = c < S ? xa(c) : INFINITY; }
+    if ((c == 0 || xl <= Q) && (c == S || !(xh <= Q))) return c;
+    int lo = 0, hi = S;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (xa(mid) <= Q) lo = mid + 1; else hi = mid;
+    }
+    c = lo;
+    xl = c > 0 ? xa(c - 1) : -INFINITY;
+    xh = c < S ? xa(c) : INFINITY;
+    return c;
+}
+
+// Persistent: each workgroup stages the tap table once, then walks tiles of
+// RP sorted pulses x RQ range bins (its pulse groups in turn, each swept
+// along range so strip halos are L2 hits).  Per tile: every pulse's source
+// strip goes to LDS while each thread finds the FP64 brackets of its 8 bins
+// (closed-form guess from the uniform frequency grid, exact comparisons);
+// then four-bin shared-window sinc sums (or np.interp) and a transpose
+// through LDS so the corner-turned g1t rows are written 128 bytes at a time.
+// Lane layout: thread t serves pulse t % 16 and bin groups t / 16 and
+// t / 16 + 32, so the lanes of a wave read 16 different strips.
+template <bool SINC, bool FULLFREQ>
+__global__ __launch_bounds__(RT, 2) void k_range_f32(
+    const float2 *__restrict__ sig, int m, int n, const int32_t *__restrict__ order,
+    const double *__restrict__ a64, const double *__restrict__ f0v, const double *__restrict__ dfv,
+    const double *__restrict__ freq, const Grid1 KR,
+    const int16_t *__restrict__ sinc, float2 *__restrict__ g1t, int ldp,
+    unsigned *__restrict__ nslow) {
+    static_assert(sizeof(float2) * RQ * (RP + 1) <= sizeof(float2) * RP * RSP, "tile fits the strips");
+    static_assert(RQ == 4 * 2 * (RT / RP), "two groups of four bins per thread");
+    __shared__ float2 strip[RP * RSP];                 // source strips, then the output tile
+    __shared__ float tapl[SINC ? TROWS * 256 : 1];
+    __shared__ int lo_s[RP], hi_s[RP], st_s[RP], dir_s[RP];
+    auto tile = reinterpret_cast<float2 (*)[RP + 1]>(&strip[0]);
+    if (SINC) load_taps(tapl, sinc);
+    const int tiles_p = (m + RP - 1) / RP, tiles_q = (n + RQ - 1) / RQ;
+    const int my_groups = blockIdx.x < tiles_p ? (tiles_p - 1 - blockIdx.x) / gridDim.x + 1 : 0;
+    unsigned slow = 0;
+    for (int it = 0; it < my_groups * tiles_q; ++it) {
+        const int p0 = (blockIdx.x + (it / tiles_q) * gridDim.x) * RP, q0 = (it % tiles_q) * RQ;
+        const int qe = min(q0 + RQ, n);
+        int t = threadIdx.x;                          // opaque: per-thread values stay in the loop
+        asm volatile("" : "+v"(t));
+        const int pl = t % RP, gi = t / RP;           // pulse in tile, bin group
+        const int p = p0 + pl;
+        // per-pulse position model kr(j) = (2 pr / c) f(j), f(j) = f0 + j df
+        int i = 0;
+        double ai = 1.0, f0 = 0.0, df = 1.0;
+        if (p < m) {
+            i = order[p];
+            ai = a64[i];
+            if (FULLFREQ) {
+                f0 = freq[(size_t)i * n];
+                df = __ddiv_rn(__dsub_rn(freq[(size_t)i * n + n - 1], f0), (double)(n - 1));
+            } else {
+                f0 = f0v[i];
+                df = dfv[i];
+            }
+        }
+        const double *frow = freq + (size_t)i * n;
+        auto kr = [&](int j) {
+            const double f = FULLFREQ ? frow[j] : __dadd_rn(f0, __dmul_rn((double)j, df));
+            return __dmul_rn(ai, f);
+        };
+        const bool asc = p < m ? !(kr(0) > kr(n - 1)) : true;
+        auto xa = [&](int u) { return kr(asc ? u : n - 1 - u); };
+        const double x0 = ai * f0, inv_adf = 1.0 / (ai * df);
+        auto guess = [&](double Q) {                  // ascending-view count estimate
+            const double js = (Q - x0) * inv_adf;
+            const double g = asc ? js : (double)(n - 1) - js;
+            return (g == g) ? (int)fmin(fmax(floor(g) + 1.0, 0.0), (double)n) : 0;
+        };
+        const float2 *row = sig + (size_t)i * n;
+        auto fa_g = [&](int u) { return row[asc ? u : n - 1 - u]; };
+        __syncthreads();                              // previous tile fully consumed
+        if (t < RP) {                                 // pulse t's strip: first window on, RS samples
+            int lo = 0, hi = 0;
+            if (p < m) {
+                double xl, xh;
+                const double Q = KR(q0);
+                const int c = bracket_from(xa, n, Q, guess(Q), xl, xh);
+                lo = max(0, min(c - 1 - 15 - 2, n - RS));
+                hi = min(lo + RS, n);
+            }
+            lo_s[t] = lo;
+            hi_s[t] = hi;
+            st_s[t] = asc ? lo : n - 1 - lo;          // sample index of strip slot 0
+            dir_s[t] = asc ? 1 : -1;
+        }
+        __syncthreads();
+        // strip loads in flight while the brackets are found
+        float2 pre[RLD];
+#pragma unroll
+        for (int k = 0; k < RLD; ++k) {
+            const int r = t + k * RT;
+            const int pp = r / RS, u = r - (r / RS) * RS;
+            pre[k] = make_float2(0.f, 0.f);
+            if (pp < RP && lo_s[pp] + u < hi_s[pp])
+                pre[k] = ld_stream(sig + (size_t)order[p0 + pp] * n + st_s[pp] + dir_s[pp] * u);
+        }
+        // brackets of this thread's bins: (c << 8) | phase and the fraction
+        int code[8];
+        float frq[8];
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int k = g * 4 + u;
+                const int q = q0 + 4 * (gi + g * (RT / RP)) + u;
+                code[k] = 0;
+                frq[k] = 0.f;
+                if (p >= m || q >= qe) continue;
+                const double Q = KR(q);
+                double xl, xh;
+                const int c = bracket_from(xa, n, Q, guess(Q), xl, xh);
+                int ph = 0;
+                if (c == n) {
+                    ph = (Q == xl) ? 1 : 0;           // last sample, or outside
+                } else if (c > 0) {
+                    const double fr = __ddiv_rn(__dsub_rn(Q, xl), __dsub_rn(xh, xl));
+                    ph = (int)fmin(rint(fr * 32768.0), 32767.0) >> 7;
+                    frq[k] = (float)fr;
+                }
+                code[k] = (c << 8) | ph;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < RLD; ++k) {
+            const int r = t + k * RT;
+            const int pp = r / RS, u = r - (r / RS) * RS;
+            if (pp < RP) strip[pp * RSP + u] = pre[k];
+        }
+        __syncthreads();
+        // gather
+        float2 v[8];
+        const int lo = lo_s[pl], hi = hi_s[pl];
+        const float2 *sp = strip + pl * RSP;
+        auto fa = [&](int u) { return (u >= lo && u < hi) ? sp[u - lo] : fa_g(u); };
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+            int d[4], ph[4];
+            const int J0 = (code[g * 4] >> 8) - 1;
+            bool fast = SINC && p < m;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int J = (code[g * 4 + u] >> 8) - 1;
+                d[u] = J - J0;
+                ph[u] = code[g * 4 + u] & 255;
+                fast = fast && J >= 15 && J <= n - 17 && d[u] >= 0 && d[u] <= KD &&
+                       q0 + 4 * (gi + g * (RT / RP)) + u < qe;
+            }
+            fast = fast && J0 - 15 >= lo && J0 - 15 + KW <= hi;
+            if (fast) {
+                float2 w4[4];
+                sinc4u(sp, J0 - 15 - lo, tapl, ph, d, w4);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) v[g * 4 + u] = w4[u];
+            } else {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int k = g * 4 + u;
+                    const int q = q0 + 4 * (gi + g * (RT / RP)) + u;
+                    const int c = code[k] >> 8;
+                    float2 r = make_float2(0.f, 0.f);
+                    if (p < m && q < qe && c > 0) {
+                        if (c == n) {
+                            if (code[k] & 255) r = fa(n - 1);
+                        } else if (SINC && c - 1 >= 15 && c - 1 <= n - 17) {
+                            ++slow;
+                            r = sinc1(fa, c - 1, code[k] & 255, tapl);
+                        } else {
+                            const float2 a0 = fa(c - 1), a1 = fa(c);
+                            const float fr = frq[k];
+                            r = make_float2(fmaf(a1.x - a0.x, fr, a0.x), fmaf(a1.y - a0.y, fr, a0.y));
+                        }
+                    }
+                    v[k] = r;
+                }
+            }
+        }
+        __syncthreads();                              // strips dead: the tile takes over
+#pragma unroll
+        for (int g = 0; g < 2; ++g)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) tile[4 * (gi + g * (RT / RP)) + u][pl] = v[g * 4 + u];
+        __syncthreads();
+        for (int r = t; r < RQ * RP; r += RT) {
+            const int qq = r / RP, pp = r % RP;
+            if (q0 + qq < n && p0 + pp < m) st_stream(g1t + (size_t)(q0 + qq) * ldp + p0 + pp, tile[qq][pp]);
+        }
+    }
+    count_wave(nslow, slow);
+}
+
+// ------------------------------------------------------------------ pass 2 (+ window + FFT A)
+// A is row-major [range bin j][cross r]: every workgroup of the azimuth
+// kernel writes whole 128-byte lines.
+__device__ __forceinline__ size_t ablk(int j, int r, int m2) { return (size_t)j * m2 + r; }
+
+// Workgroup b -> consecutive items on the same XCD (b % 8 selects the XCD).
+__device__ __forceinline__ int xcd_map(int b, int count) {
+    const int per = (count + 7) / 8;
+    return (b % 8) * per + b / 8;
+}
+
+// Elements per thread of the azimuth kernel's FFT and resample: 8, so an
+// 8192-point line runs on 1024 threads (16 waves keep the CU busy while one
+// workgroup holds the LDS).
+template <int L>
+constexpr int az_emax() { return 8; }
+template <int L>
+constexpr int az_nb() { return pfa32::Cfg<L, az_emax<L>()>::NB; }
+
+// Bracket-guess table over tan(phi): idx[b] = #{p : tan_s[p] < lo + b / inv_w}.
+struct TanBins {
+    const int32_t *idx;
+    double lo, inv_w;
+    int bins;
+    __device__ __forceinline__ int guess(double t) const {
+        const double b = floor(__dmul_rn(__dsub_rn(t, lo), inv_w));
+        return idx[(b == b) ? (int)fmin(fmax(b, 0.0), (double)bins) : 0];
+    }
+};
+
+// Persistent over range bins: iteration it of workgroup b (XCD x = b % 8, rank
+// b / 8) takes j = x * ceil(n/8) + it * (grid/8) + rank.  One LDS region holds,
+// in turn, the line's FP64 source positions KR_j tan_s[p] (bracket search), the
+// sorted-pulse row (the gather's source, prefetched into registers during the
+// previous line) and the windowed, sign-folded line the FFT runs on.
+// Thread t owns outputs 4 (g NB + t) + u, u < 4: the four share one sinc window.
+// MODE 0: store the windowed k-space g2[q][j] (resample_kspace stage output)
+// MODE 1: fused -- window, sign fold, pad, FFT along cross -> A (row-major)
+// MODE 2: FFT along cross of a given g2 (focus() stage) -> A (row-major)
+// FFT form of the line transform: 0 FP32 vector FFT (A complex64); 1 BF16,
+// 2 FP16, 3 BF16 with hi/lo DFT matrices -- matrix-core FFT (pfa_fft_mfma.hpp),
+// A stored as packed 16-bit pairs with a per-row exponent.
+constexpr int FK_F32 = 0, FK_BF16 = 1, FK_FP16 = 2, FK_BF16S = 3;
+template <int L, int FK>
+constexpr int optab_len() {
+    if constexpr (FK == FK_F32) return 1;
+    else return 2 * pfam::ntiles<L>() * 64;
+}
+
+template <int L, int MODE, bool SINC, bool INV, int FK = FK_F32>
+__global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
+    const float2 *__restrict__ g1t, int ldp, int m, int n, const Grid1 KR,
+    const double *__restrict__ tan_s, const TanBins tb, const Grid1 KC,
+    const double *__restrict__ hm, const double *__restrict__ hn,
+    const int16_t *__restrict__ sinc, const float2 *__restrict__ g2in,
+    float2 *__restrict__ out, const float2 *__restrict__ twg, unsigned *__restrict__ nslow,
+    unsigned *__restrict__ A16, int *__restrict__ expA) {
+    static_assert(FK == FK_F32 || MODE == 1, "matrix-core FFT only in the fused mode");
+    constexpr bool BF = FK == FK_BF16 || FK == FK_BF16S;
+    constexpr bool SPLIT = FK == FK_BF16S;
+    constexpr int EM = az_emax<L>();
+    using Cf = pfa32::Cfg<L, EM>;
+    constexpr int M2 = Cf::N;
+    constexpr int NB = Cf::NB;
+    constexpr int G = (M2 + 4 * NB - 1) / (4 * NB);   // 4-output groups per thread
+    constexpr int PR = (M2 + NB - 1) / NB;             // row samples per thread
+    constexpr bool RES = MODE != 2;                    // resampling modes
+    __shared__ float2 reg[Cf::PADN + 64];              // positions | row | FFT line
+    __shared__ float tapl[(SINC && RES) ? TROWS * 256 : 1];
+    __shared__ float2 twl[M2 / 64 + 64];
+    __shared__ pfam::u16x8 optab[optab_len<L, FK>()];  // DFT-matrix operands (matrix-core FFT)
+    __shared__ float wmax[NB / 64];
+    double *xs = reinterpret_cast<double *>(reg);
+    const int t = threadIdx.x;
+    if (SINC && RES) load_taps(tapl, sinc);
+    const pfa32::Tw tw = pfa32::load_tw<M2>(twl, twg);
+    if constexpr (FK != FK_F32) pfam::build_ops_stage<L, BF, INV, 0>(optab);
+    const int per_x = (n + 7) / 8;
+    const int ranks = (gridDim.x + 7) / 8;
+    const int x = blockIdx.x % 8, rank = blockIdx.x / 8;
+    auto line_of = [&](int base) { return x * per_x + base + rank; };
+    auto active_at = [&](int base) { return base + rank < per_x && line_of(base) < n; };
+    unsigned slow = 0;
+    float2 pre[PR];                                    // next row, in flight
+    auto prefetch = [&](int j) {
+        const float2 *row = g1t + (size_t)j * ldp;
+#pragma unroll
+        for (int k = 0; k < PR; ++k) {
+            const int p = t + k * NB;
+            pre[k] = p < m ? ld_stream(row + p) : make_float2(0.f, 0.f);
+        }
+    };
+    if (RES && active_at(0)) prefetch(line_of(0));
+    for (int base = 0; base < per_x; base += ranks) {
+        if (!active_at(base)) continue;               // uniform per workgroup
+        const int j = line_of(base);
+        // per-output values (KC(q), tapers) are recomputed per line: hoisted
+        // out of the loop they would pin ~40 registers for its whole length
+        int tl = t;
+        asm volatile("" : "+v"(tl));
+        float2 v[G * 4];
+        if (MODE == 2) {
+            __syncthreads();                          // previous line fully consumed
+            for (int q = t; q < M2; q += NB) {
+                float2 w = q < m ? g2in[(size_t)q * n + j] : make_float2(0.f, 0.f);
+                if ((q + j) & 1) w = make_float2(-w.x, -w.y);
+                reg[pfa32::padf(q)] = w;
+            }
+        } else {
+            const double krj = KR(j);
+            const double inv_krj = 1.0 / krj;
+            // (1) FP64 source positions of this line
+            __syncthreads();                          // previous line fully consumed
+            for (int p = t; p < m; p += NB) xs[p] = __dmul_rn(krj, tan_s[p]);
+            __syncthreads();
+            // (2) brackets: c = #{p : xs[p] <= Q} (np.interp / searchsorted-right),
+            // kept as (c << 8) | sinc phase, and the interval fraction
+            int code[G * 4];
+            float frq[G * 4];
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int q = (g * NB + tl) * 4 + u;
+                    const int k = g * 4 + u;
+                    code[k] = 0;
+                    frq[k] = 0.f;
+                    if (q >= m) continue;
+                    const double Q = KC(q);
+                    int c = tb.guess(__dmul_rn(Q, inv_krj));
+                    double xl = c > 0 ? xs[c - 1] : -INFINITY;
+                    double xh = c < m ? xs[c] : INFINITY;
+                    if (xh <= Q) {                    // the guess is low by one for ~1/4 of outputs
+                        ++c;
+                        xl = xh;
+                        xh = c < m ? xs[c] : INFINITY;
+                    }
+                    if (!(xl <= Q) || xh <= Q) {
+                        ++slow;
+                        while (c > 0 && !(xl <= Q)) { --c; xh = xl; xl = c > 0 ? xs[c - 1] : -INFINITY; }
+                        while (c < m && xh <= Q) { ++c; xl = xh; xh = c < m ? xs[c] : INFINITY; }
+                    }
+                    int ph = 0;
+                    if (c == m) {
+                        ph = (Q == xl) ? 1 : 0;                              // last sample, or outside
+                    } else if (c > 0) {
+                        const double fr = __ddiv_rn(__dsub_rn(Q, xl), __dsub_rn(xh, xl));
+                        ph = (int)fmin(rint(fr * 32768.0), 32767.0) >> 7;
+                        frq[k] = (float)fr;
+                    }
+                    code[k] = (c << 8) | ph;
+                }
+            }
+            __syncthreads();                          // positions dead: the row takes over
+            // (3) the row (unpadded), zero past m so shifted windows read zeros
+#pragma unroll
+            for (int k = 0; k < PR; ++k) {
+                const int p = t + k * NB;
+                if (p < M2) reg[p] = pre[k];
+            }
+            if (t < 48) reg[m + t] = make_float2(0.f, 0.f);
+            __syncthreads();
+            // (4) gather
+            auto fa = [&](int u) { return reg[u]; };
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                int d[4], ph[4];
+                bool fast = SINC;
+                const int J0 = (code[g * 4] >> 8) - 1;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int J = (code[g * 4 + u] >> 8) - 1;
+                    d[u] = J - J0;
+                    ph[u] = code[g * 4 + u] & 255;
+                    fast = fast && J >= 15 && J <= m - 17 && d[u] >= 0 && d[u] <= KD;
+                }
+                if (fast) {
+                    float2 w4[4];
+                    sinc4u(reg, J0 - 15, tapl, ph, d, w4);
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) v[g * 4 + u] = w4[u];
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int k = g * 4 + u;
+                        const int q = (g * NB + tl) * 4 + u;
+                        const int c = code[k] >> 8;
+                        float2 r = make_float2(0.f, 0.f);
+                        if (q < m && c > 0) {
+                            if (c == m) {
+                                if (ph[u]) r = fa(m - 1);
+                            } else if (SINC && c - 1 >= 15 && c - 1 <= m - 17) {
+                                r = sinc1(fa, c - 1, ph[u], tapl);
+                            } else {
+                                const float2 f0 = fa(c - 1), f1 = fa(c);
+                                const float fr = frq[k];
+                                r = make_float2(fmaf(f1.x - f0.x, fr, f0.x), fmaf(f1.y - f0.y, fr, f0.y));
+                            }
+                        }
+                        v[k] = r;
+                    }
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            {                                         // the next row, in flight during the FFT
+                const int nb2 = base + ranks;
+                if (nb2 < per_x && active_at(nb2)) prefetch(line_of(nb2));
+            }
+            // window
+            const double hj = hn ? hn[j] : 1.0;
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int q = (g * NB + tl) * 4 + u;
+                    if (q < m && hm) {
+                        const float wgt = (float)__dmul_rn(hm[q], hj);
+                        v[g * 4 + u] = make_float2(v[g * 4 + u].x * wgt, v[g * 4 + u].y * wgt);
+                    }
+                }
+            }
+            if (MODE == 0) {
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int q = (g * NB + tl) * 4 + u;
+                        if (q < m) out[(size_t)q * n + j] = v[g * 4 + u];
+                    }
+                }
+                continue;
+            }
+            if constexpr (FK != FK_F32) {
+                // block scaling: the line's largest component sets a power-of-two exponent
+                float mx = 0.f;
+#pragma unroll
+                for (int k = 0; k < G * 4; ++k) mx = fmaxf(mx, fmaxf(fabsf(v[k].x), fabsf(v[k].y)));
+                for (int off = 32; off >= 1; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+                if ((t & 63) == 0) wmax[t >> 6] = mx;
+                __syncthreads();                      // the row is dead: the FFT planes take over
+                float lm = 0.f;
+#pragma unroll
+                for (int i = 0; i < NB / 64; ++i) lm = fmaxf(lm, wmax[i]);
+                const int se = pfam::norm_exp(lm);
+                const float sc = ldexpf(1.0f, -se);
+                unsigned short *planes = reinterpret_cast<unsigned short *>(reg);
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int q = (g * NB + tl) * 4 + u;
+                        if (q < M2) {
+                            float2 w = make_float2(v[g * 4 + u].x * sc, v[g * 4 + u].y * sc);
+                            if ((q + j) & 1) w = make_float2(-w.x, -w.y);
+                            pfam::put<L, BF>(planes, 0, q, w);
+                        }
+                    }
+                }
+                __syncthreads();
+                constexpr int NW = pfam::nw_for<L, 1, NB / 64>();
+                unsigned *dst = A16 + (size_t)j * M2;
+                pfam::fft<L, BF, INV, SPLIT, 1, NW>(planes, tw, pfam::OpsLds<L, BF, INV>{optab},
+                    [&](int, int o, float2 y) {
+                        const unsigned pk = (unsigned)pfam::to16<BF>(y.x) | ((unsigned)pfam::to16<BF>(y.y) << 16);
+                        __builtin_nontemporal_store(pk, dst + o);
+                    });
+                if (t == 0) expA[j] = se + (INV ? 0 : L);
+                continue;
+            }
+            __syncthreads();                          // the row is dead: the FFT line takes over
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int q = (g * NB + tl) * 4 + u;
+                    if (q < M2) {
+                        float2 w = v[g * 4 + u];
+                        if ((q + j) & 1) w = make_float2(-w.x, -w.y);
+                        reg[pfa32::padf(q)] = w;
+                    }
+                }
+            }
+        }
+        if constexpr (MODE != 0) {
+            __syncthreads();
+            float2 *dst = out + ablk(j, 0, M2);
+            const float scale = INV ? 1.0f / (float)M2 : 1.0f;
+            pfa32::fft<L, INV, EM>(reg, tw, [&](int idx, float2 w) {
+                if (INV) w = make_float2(w.x * scale, w.y * scale);
+                st_stream(dst + idx, w);
+            });
+        }
+    }
+    count_wave(nslow ? nslow + 1 : nullptr, slow);
+}
+
+// ------------------------------------------------------------------ FFT B (range) + detect
+// OUT 0: complex64 image, 1: |image| float32.  Workgroup b -> cross bin r with
+// consecutive bins on the same XCD.
+template <int L, bool INV, int OUT>
+__global__ __launch_bounds__(pfa32::Cfg<L>::NT < 64 ? 64 : pfa32::Cfg<L>::NT) void k_fft_b_f32(
+    const float2 *__restrict__ A, int n, int m2, void *__restrict__ outp,
+    const float2 *__restrict__ twg) {
+    using Cf = pfa32::Cfg<L>;
+    constexpr int N2 = Cf::N;
+    __shared__ float2 buf[Cf::PADN];
+    __shared__ float2 twl[N2 / 64 + 64];
+    const pfa32::Tw tw = pfa32::load_tw<N2>(twl, twg);
+    const int r = xcd_map(blockIdx.x, m2);
+    const float scale = INV ? 1.0f / (float)N2 : 1.0f;
+    // the first pass reads its butterflies straight from A (the corner-turn)
+    auto src = [&](int jj) { return jj < n ? A[ablk(jj, r, m2)] : make_float2(0.f, 0.f); };
+    pfa32::fft_src<L, INV>(buf, tw, src, [&](int c, float2 v) {
+        if (INV) v = make_float2(v.x * scale, v.y * scale);
+        if ((r + c) & 1) v = make_float2(-v.x, -v.y);
+        if (OUT == 0) reinterpret_cast<float2 *>(outp)[(size_t)r * N2 + c] = v;
+        else reinterpret_cast<float *>(outp)[(size_t)r * N2 + c] = __fsqrt_rn(v.x * v.x + v.y * v.y);
+    });
+}
+
+// Two-level twiddle tables of an n-point line: hi[k] = W^(64 k), k < n/64,
+// then lo[k] = W^k, k < 64, with W = exp(-2 pi i / n), rounded from double.
+const float2 *twiddles_f32(pfa_ctx *ctx, int n, int *st) {
+    char slot[32];
+    snprintf(slot, sizeof slot, "twf2_%d", n);
+    auto it = ctx->ws.find(slot);
+    if (it != ctx->ws.end() && it->second.bytes) return (const float2 *)it->second.ptr;
+    std::vector<float2> h((size_t)n / 64 + 64);
+    for (int k = 0; k < n / 64; ++k) {
+        const double th = 2.0 * M_PI * (double)(64 * k) / (double)n;
+        h[k] = make_float2((float)std::cos(th), (float)-std::sin(th));
+    }
+    for (int k = 0; k < 64; ++k) {
+        const double th = 2.0 * M_PI * (double)k / (double)n;
+        h[n / 64 + k] = make_float2((float)std::cos(th), (float)-std::sin(th));
+    }
+    void *d = pfa_ws(ctx, slot, h.size() * sizeof(float2), st);
+    if (!d) return nullptr;
+    if (hipMemcpy(d, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess) {
+        *st = pfa_fail(ctx, PFA_ERR_HIP, "twiddle upload failed");
+        return nullptr;
+    }
+    return (const float2 *)d;
+}
+
+Grid1 grid_of(const double *tab, const double *lin, int cnt) {
+    if (lin) return Grid1{nullptr, lin[0], lin[1], lin[2], cnt};
+    return Grid1{tab, 0.0, 0.0, 0.0, cnt};
+}
+
+int log2_exact(int n) {
+    int l = 0;
+    while ((1 << l) < n) ++l;
+    return (1 << l) == n ? l : -1;
+}
+
+template <int L>
+constexpr int nb_of() { return pfa32::Cfg<L>::NT < 64 ? 64 : pfa32::Cfg<L>::NT; }
+
+}  // namespace

@@ -106,3 +106,8 @@ int launch_float_range(pfa_plan *p, const void *sig, const double *freq, void *g
 int launch_float_azimuth(pfa_plan *p, int mode, const void *g1t, int ldp, const void *g2in,
                          void *out);
 int launch_float_fft_b(pfa_plan *p, const void *A, void *out, int out_kind);
+// matrix-core FFT form; fk: 1 BF16, 2 FP16, 3 BF16 with hi/lo DFT matrices
+int launch_float_azimuth_mfma(pfa_plan *p, const void *g1t, int ldp, unsigned *A16, int *expA,
+                              int fk);
+int launch_float_fft_b_mfma(pfa_plan *p, const unsigned *A16, const int *expA, void *out,
+                            int out_kind, int fk);

@@ -1,0 +1,155 @@
+// pfa_mfma.hip -- the precision form of the float focus: both 1-D FFT passes as
+// dense DFT-matrix stages on the gfx950 matrix cores (BF16 or FP16 operands,
+// FP32 accumulation, per-line power-of-two block scaling, optional hi/lo DFT
+// matrices; pfa_fft_mfma.hpp), with the intermediate A stored as packed
+// 16-bit complex pairs plus one exponent per range row.  The resample passes
+// are the FP32 kernels of pfa_float_dev.hpp.
+//
+// Kernels
+//   k_azimuth_f32<.., FK>  azimuth pass + window + sign fold (FP32), then the
+//                          FFT along cross on the matrix cores -> A16[j][r], expA[j]
+//   k_fft_b_mfma           FFT along range for 4 adjacent cross bins per
+//                          workgroup (16-byte row loads of A16 = the corner-turn),
+//                          output sign fold and detection in the store
+#include "pfa_float_dev.hpp"
+
+#include <climits>
+
+namespace {
+
+template <int L>
+constexpr int fftb_mfma_nb() { return 64 * pfam::nw_for<L, 4, 8>(); }
+
+// OUT 0: complex64 image, 1: |image| float32; image [cross r][range c].
+template <int L, bool INV, int OUT, int FK>
+__global__ __launch_bounds__(fftb_mfma_nb<L>()) void k_fft_b_mfma(
+    const unsigned *__restrict__ A16, const int *__restrict__ expA, int n, int m2,
+    void *__restrict__ outp, const float2 *__restrict__ twg) {
+    constexpr bool BF = FK == FK_BF16 || FK == FK_BF16S;
+    constexpr bool SPLIT = FK == FK_BF16S;
+    constexpr int N2 = 1 << L, NCOL = 4, NB = fftb_mfma_nb<L>(), BW = NB / 64;
+    constexpr int NW = pfam::nw_for<L, NCOL, BW>();
+    constexpr int PL = pfam::plane_len<L>();
+    __shared__ unsigned short planes[NCOL * 2 * PL];
+    __shared__ float2 twl[N2 / 64 + 64];
+    __shared__ int emx[BW];
+    const pfa32::Tw tw = pfa32::load_tw<N2>(twl, twg);
+    const int t = threadIdx.x;
+    const int r0 = xcd_map(blockIdx.x, m2 / NCOL) * NCOL;
+    // common exponent of the block: |A16| < 2^2, so 2^-(max expA + 2) bounds every input by 1
+    int e = INT_MIN;
+    for (int jj = t; jj < n; jj += NB) e = max(e, expA[jj]);
+    for (int off = 32; off >= 1; off >>= 1) e = max(e, __shfl_xor(e, off, 64));
+    if ((t & 63) == 0) emx[t >> 6] = e;
+    __syncthreads();
+    int S = INT_MIN;
+#pragma unroll
+    for (int i = 0; i < BW; ++i) S = max(S, emx[i]);
+    S = (S == INT_MIN) ? 0 : S + 2;
+    // the corner-turn: 4 adjacent cross bins of every range row, rescaled to 2^-S
+    for (int jj = t; jj < N2; jj += NB) {
+        uint4 a = make_uint4(0u, 0u, 0u, 0u);
+        float sc = 0.f;
+        if (jj < n) {
+            a = *reinterpret_cast<const uint4 *>(A16 + (size_t)jj * m2 + r0);
+            sc = ldexpf(1.0f, expA[jj] - S);
+        }
+        const unsigned wv[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+        for (int c = 0; c < NCOL; ++c) {
+            const float re = pfam::from16<BF>((unsigned short)(wv[c] & 0xffffu));
+            const float im = pfam::from16<BF>((unsigned short)(wv[c] >> 16));
+            pfam::put<L, BF>(planes, c, jj, make_float2(re * sc, im * sc));
+        }
+    }
+    __syncthreads();
+    const float osc = ldexpf(1.0f, S + (INV ? 0 : L));
+    pfam::fft<L, BF, INV, SPLIT, NCOL, NW>(planes, tw, pfam::OpsReg<L, BF, INV>{},
+        [&](int col, int c, float2 y) {
+            const int r = r0 + col;
+            float2 v = make_float2(y.x * osc, y.y * osc);
+            if ((r + c) & 1) v = make_float2(-v.x, -v.y);
+            if (OUT == 0) reinterpret_cast<float2 *>(outp)[(size_t)r * N2 + c] = v;
+            else reinterpret_cast<float *>(outp)[(size_t)r * N2 + c] = __fsqrt_rn(v.x * v.x + v.y * v.y);
+        });
+}
+
+}  // namespace
+
+#define PFA_MFMA_SWITCH(L, MACRO)                                                                   \
+    switch (L) {                                                                                  \
+        MACRO(8) MACRO(9) MACRO(10) MACRO(12) MACRO(13)                                           \
+        default: return pfa_fail(ctx, PFA_ERR_DIMS, "matrix-core FFT sizes are 256, 512, 1024, 4096 and 8192"); \
+    }
+
+int launch_float_azimuth_mfma(pfa_plan *p, const void *g1t, int ldp, unsigned *A16, int *expA,
+                              int fk) {
+    pfa_ctx *ctx = p->ctx;
+    int st = PFA_OK;
+    const int m2 = p->m2;
+    const float2 *tw = twiddles_f32(ctx, m2, &st);
+    if (!tw) return st;
+    const bool sinc = p->flags & PFA_FLOAT_SINC32;
+    const bool inv = p->sgn > 0;
+    const int L = log2_exact(m2);
+    const int grid_az = std::max(8, std::min((p->n + 7) / 8 * 8, ctx->num_cu / 8 * 8));
+    const TanBins tb{p->tan_bin, p->tb_lo, p->tb_inv_w, p->tb_bins};
+    pfa_timer_begin(ctx, PFA_TIMER_AZIMUTH);
+#define ARGS (const float2 *)g1t, ldp, p->m, p->n, grid_of(p->KR, p->kr_lin, p->n), p->tan_s64, tb, \
+             grid_of(p->KC, p->kc_lin, p->m), p->hm, p->hn, p->sinc, nullptr, nullptr, tw,          \
+             (unsigned *)(ctx->d_counters + 32), A16, expA
+#define K(LL, S, I, F) hipLaunchKernelGGL((k_azimuth_f32<LL, 1, S, I, F>), dim3(grid_az), dim3(az_nb<LL>()), 0, ctx->stream, ARGS)
+#define KF(LL, F)                                 \
+    if (sinc && inv) K(LL, true, true, F);        \
+    else if (sinc) K(LL, true, false, F);         \
+    else if (inv) K(LL, false, true, F);          \
+    else K(LL, false, false, F);
+#define LAUNCH(LL)                                 \
+    case LL:                                       \
+        if (fk == FK_BF16) { KF(LL, FK_BF16) }     \
+        else if (fk == FK_FP16) { KF(LL, FK_FP16) } \
+        else { KF(LL, FK_BF16S) }                  \
+        break;
+    PFA_MFMA_SWITCH(L, LAUNCH)
+#undef LAUNCH
+#undef KF
+#undef K
+#undef ARGS
+    pfa_timer_end(ctx, PFA_TIMER_AZIMUTH);
+    PFA_HIP(ctx, hipGetLastError());
+    return PFA_OK;
+}
+
+int launch_float_fft_b_mfma(pfa_plan *p, const unsigned *A16, const int *expA, void *out,
+                            int out_kind, int fk) {
+    pfa_ctx *ctx = p->ctx;
+    int st = PFA_OK;
+    const int n2 = p->n2;
+    const float2 *tw = twiddles_f32(ctx, n2, &st);
+    if (!tw) return st;
+    const bool inv = p->sgn > 0;
+    const int L = log2_exact(n2);
+    const dim3 grid(p->m2 / 4);
+    pfa_timer_begin(ctx, PFA_TIMER_FFT_B);
+#define ARGS A16, expA, p->n, p->m2, out, tw
+#define K(LL, I, O, F) hipLaunchKernelGGL((k_fft_b_mfma<LL, I, O, F>), grid, dim3(fftb_mfma_nb<LL>()), 0, ctx->stream, ARGS)
+#define KF(LL, F)                              \
+    if (inv && out_kind) K(LL, true, 1, F);    \
+    else if (inv) K(LL, true, 0, F);           \
+    else if (out_kind) K(LL, false, 1, F);     \
+    else K(LL, false, 0, F);
+#define LAUNCH(LL)                                 \
+    case LL:                                       \
+        if (fk == FK_BF16) { KF(LL, FK_BF16) }     \
+        else if (fk == FK_FP16) { KF(LL, FK_FP16) } \
+        else { KF(LL, FK_BF16S) }                  \
+        break;
+    PFA_MFMA_SWITCH(L, LAUNCH)
+#undef LAUNCH
+#undef KF
+#undef K
+#undef ARGS
+    pfa_timer_end(ctx, PFA_TIMER_FFT_B);
+    PFA_HIP(ctx, hipGetLastError());
+    return PFA_OK;
+}

@@ -24,7 +24,12 @@ from .tables import sinc_table_q15
 
 C_LIGHT = 299792458.0
 FLOAT_SINC32 = 0x1
+FLOAT_FFT_BF16, FLOAT_FFT_FP16, FLOAT_FFT_SPLIT = 0x2, 0x4, 0x8
 OUT_C64, OUT_ABS_F32 = 0, 1
+# FFT forms of the focus: the FP32 vector FFT (the reference's arithmetic) or
+# the matrix-core DFT stages with 16-bit operands (precision form)
+FFT_FORMS = {"fp32": 0, "bf16": FLOAT_FFT_BF16, "fp16": FLOAT_FFT_FP16,
+             "bf16-split": FLOAT_FFT_BF16 | FLOAT_FFT_SPLIT}
 
 
 def to_pow2(n: int) -> int:
@@ -78,9 +83,12 @@ class FloatPlan:
     """A float-mode plan: geometry tables resident on the device."""
 
     def __init__(self, m, n, sgn, geom, f0=None, df=None, window=True, interp="linear",
-                 device=0):
+                 device=0, fft="fp32"):
         if interp not in ("linear", "sinc32"):
             raise PfaError(-1, f"unknown interpolator {interp!r}")
+        if fft not in FFT_FORMS:
+            raise PfaError(-1, f"unknown FFT form {fft!r} (one of {sorted(FFT_FORMS)})")
+        self.fft = fft
         self.ctx = context(device)
         self.m, self.n, self.sgn = m, n, sgn
         self.m2, self.n2 = max(64, to_pow2(m)), max(64, to_pow2(n))
@@ -101,7 +109,7 @@ class FloatPlan:
         self.ctx.call("pfa_plan_float", C.byref(self._h), m, n, int(sgn), ptr(k["pr"]),
                       ptr(k["order"]), ptr(k["tan_s"]), ptr(k["KR"]), ptr(k["KC"]),
                       ptr(k.get("f0")), ptr(k.get("df")), ptr(k.get("hm")), ptr(k.get("hn")),
-                      ptr(sinc), FLOAT_SINC32 if interp == "sinc32" else 0)
+                      ptr(sinc), (FLOAT_SINC32 if interp == "sinc32" else 0) | FFT_FORMS[fft])
         self.geo = dict(geom["geo"])
 
     def _sig(self, sig):
@@ -164,10 +172,10 @@ class FloatPlan:
             pass
 
 
-def _plan_for(sig, freq, ax, ay, sgn, window=True, interp="linear", device=0):
+def _plan_for(sig, freq, ax, ay, sgn, window=True, interp="linear", device=0, fft="fp32"):
     m, n = np.shape(sig)
     geom = float_geometry(ax, ay, freq=freq)
-    return FloatPlan(m, n, sgn, geom, window=window, interp=interp, device=device)
+    return FloatPlan(m, n, sgn, geom, window=window, interp=interp, device=device, fft=fft)
 
 
 def resample_kspace(sig, freq, ax, ay, window=True, interp="linear", device=0):
@@ -192,9 +200,11 @@ def focus(g2, geo, sgn, device=0):
     return img, out_geo
 
 
-def pfa(sig, freq, ax, ay, sgn, window=True, interp="linear", device=0):
-    """form_image_pfa.pfa -> (img complex64 (m2, n2) centred, geo)."""
-    plan = _plan_for(sig, freq, ax, ay, sgn, window, interp, device)
+def pfa(sig, freq, ax, ay, sgn, window=True, interp="linear", device=0, fft="fp32"):
+    """form_image_pfa.pfa -> (img complex64 (m2, n2) centred, geo).  fft selects
+    the FP32 FFT (default, the reference's arithmetic) or a matrix-core form
+    ("bf16", "fp16", "bf16-split")."""
+    plan = _plan_for(sig, freq, ax, ay, sgn, window, interp, device, fft)
     img = plan.focus(sig, freq)
     geo = plan.padded_geo()
     plan.close()
@@ -206,13 +216,14 @@ class GpuBackend:
     (the pow2-padded centred image, as NumpyBackend returns)."""
     name = "gpu"
 
-    def __init__(self, interp="linear", device=0):
+    def __init__(self, interp="linear", device=0, fft="fp32"):
         self.interp = interp
         self.device = device
+        self.fft = fft
 
     def focus(self, signal, tables):
         plan = _plan_for(signal, tables["freq"], tables["ax"], tables["ay"], tables["sgn"],
-                         True, self.interp, self.device)
+                         True, self.interp, self.device, self.fft)
         out = plan.focus(signal, tables["freq"], detect=True)
         plan.close()
         return out
@@ -224,6 +235,8 @@ def make_backend(name, ref_module=None):
         return GpuBackend("linear")
     if name == "gpu-sinc32":
         return GpuBackend("sinc32")
+    if name in ("gpu-bf16", "gpu-fp16"):
+        return GpuBackend("sinc32", fft=name[4:])
     if name == "numpy":
         if ref_module is None:
             raise ValueError("the numpy backend needs the reference module")
