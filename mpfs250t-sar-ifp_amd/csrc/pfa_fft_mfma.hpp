@@ -3,21 +3,23 @@
 // with FP32 accumulation.
 //
 // A radix-R stage (R = 16 or 32) of a Stockham FFT is, for each group of 16
-// vectors, the real GEMM  [Yr; Yi] = M [Xr; Xi]  with the 2R x 2R real
-// embedding of the R-point DFT,
-//     forward  M = [[C, S], [-S, C]],   inverse  M = [[C, -S], [S, C]],
-//     C[m][r] = cos(2 pi m r / R),  S[m][r] = sin(2 pi m r / R),
-// run as v_mfma_f32_16x16x32_{bf16,f16}: A = 16 rows of M (registers, built
-// once per thread), B = 32 K-values of 16 vectors (one 16-byte LDS read per
-// lane), C = 16 x 16 FP32 outputs.  Lane layouts (checked on the device):
+// vectors, the real GEMM  [Yr; Yi] = M X  with the 2R x 2R real embedding of
+// the R-point DFT; the K index of X interleaves the components,
+// X = [xr_0, xi_0, xr_1, xi_1, ...], so a lane's 8 K-values are 4 complex
+// samples stored side by side:
+//     Yr[m] = sum_r  c xr_r + s' xi_r,    Yi[m] = sum_r  -s' xr_r + c xi_r,
+//     c = cos(2 pi m r / R),  s' = sin(2 pi m r / R) (forward) or -sin (inverse),
+// run as v_mfma_f32_16x16x32_{bf16,f16}: A = 16 rows of M, B = 32 K-values of
+// 16 vectors (one 16-byte LDS read per lane), C = 16 x 16 FP32 outputs.  Lane
+// layouts (checked on the device):
 //     A[l % 16][8 (l / 16) + i],  B[8 (l / 16) + i][l % 16],  C[4 (l / 16) + i][l % 16].
 //
-// Data between stages lives in LDS as two 16-bit planes (real, imaginary),
-// each stored vector-major for the stage that reads it: element e = k + r N/R
-// of a radix-R stage sits at k R + r, padded by 8 slots every 64 so the
-// 16-byte operand reads of 16 lanes (32 or 64 bytes apart) use distinct banks.
+// Data between stages lives in LDS as 32-bit words of (re, im) 16-bit pairs,
+// stored vector-major for the stage that reads it: element e = k + r N/R of a
+// radix-R stage sits at word k R + r, padded by 4 words every 32 so the
+// 16-byte operand reads of 8 lanes (64 or 128 bytes apart) use distinct banks.
 // The twiddle of the next stage is applied in FP32 to the accumulators before
-// they are rounded and stored.
+// they are rounded and stored (one 32-bit LDS write per complex output).
 //
 // Block scaling: every stage multiplies by 1/R (exact), so the transform is
 // DFT/N; the caller normalises each line to max |component| <= 1 with a
@@ -37,6 +39,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 template <bool BF>
 struct H16 {
@@ -56,7 +59,8 @@ __device__ __forceinline__ f32x4 mma(const f16x8 &a, const f16x8 &b, const f32x4
     return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
 
-// 16-bit storage of a float (round to nearest even) and back.
+// 16-bit storage of a float (round to nearest even) and back; a complex value
+// as one 32-bit word (re in the low half).
 template <bool BF>
 __device__ __forceinline__ unsigned short to16(float x) {
     const typename H16<BF>::T h = (typename H16<BF>::T)x;
@@ -65,6 +69,14 @@ __device__ __forceinline__ unsigned short to16(float x) {
 template <bool BF>
 __device__ __forceinline__ float from16(unsigned short b) {
     return (float)__builtin_bit_cast(typename H16<BF>::T, b);
+}
+template <bool BF>
+__device__ __forceinline__ unsigned pack(float2 v) {
+    return (unsigned)to16<BF>(v.x) | ((unsigned)to16<BF>(v.y) << 16);
+}
+template <bool BF>
+__device__ __forceinline__ float2 unpack(unsigned w) {
+    return make_float2(from16<BF>((unsigned short)(w & 0xffffu)), from16<BF>((unsigned short)(w >> 16)));
 }
 
 // Radix plan: stage radices (first stage first).
@@ -98,10 +110,10 @@ constexpr int ls_of() {                      // product of the radices before st
     return p;
 }
 
-// Padded slot of position p in one 16-bit plane; plane length incl. padding.
-__device__ __forceinline__ int pp(int p) { return p + ((p >> 6) << 3); }
+// Padded word of position p in a line; line length in words incl. padding.
+__device__ __forceinline__ int pp(int p) { return p + ((p >> 5) << 2); }
 template <int L>
-constexpr int plane_len() { return (1 << L) + ((1 << L) >> 3); }
+constexpr int line_words() { return (1 << L) + ((1 << L) >> 3); }
 
 // Position of element e in the plane layout read by stage S.
 template <int L, int S>
@@ -130,7 +142,7 @@ __device__ __forceinline__ void dft_operand(int mt, int ks, int l, typename H16<
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
         const int col = 32 * ks + 8 * (l >> 4) + i;
-        const int bc = col / R, rr = col % R;
+        const int bc = col & 1, rr = col >> 1;          // interleaved (re, im) K order
         const int t = (mm * rr) % R * (32 / R);
         const float c = cos32(t), s = sin32(t);
         float v;
@@ -202,13 +214,12 @@ __device__ __forceinline__ OpsLds<L, BF, INV> build_ops(u16x8 *tab) {
     return OpsLds<L, BF, INV>{tab};
 }
 
-// One stage over NCOL lines held in `planes` (per line: re plane then im
-// plane, plane_len<L>() slots each).  Stage S reads its layout, applies the
+// One stage over NCOL lines held in `planes` (line_words<L>() words per line).  Stage S reads its layout, applies the
 // DFT, scales by 1/R, then either (not last) multiplies by the next stage's
 // twiddle and writes the next layout, or (last) hands (line, index, value) to
 // sink.  NW waves share the work; results wait in registers for the barrier.
 template <int L, bool BF, bool INV, bool SPLIT, int NCOL, int NW, int S, typename Ops, typename Sink>
-__device__ __forceinline__ void stage(unsigned short *planes, const pfa32::Tw &tw, const Ops &ops,
+__device__ __forceinline__ void stage(unsigned *planes, const pfa32::Tw &tw, const Ops &ops,
                                       Sink &sink) {
     constexpr int N = 1 << L;
     constexpr int R = MPlan<L>::R[S];
@@ -219,7 +230,7 @@ __device__ __forceinline__ void stage(unsigned short *planes, const pfa32::Tw &t
     constexpr int GROUPS = NCOL * NV / 16;             // 16-vector groups over all lines
     static_assert(NW >= 1 && GROUPS % NW == 0, "groups split evenly over the waves");
     constexpr int GPW = GROUPS / NW;
-    constexpr int PL = plane_len<L>();
+    constexpr int PL = line_words<L>();
     using V = typename H16<BF>::V;
     int lane = threadIdx.x & 63;
     asm volatile("" : "+v"(lane));
@@ -237,13 +248,11 @@ __device__ __forceinline__ void stage(unsigned short *planes, const pfa32::Tw &t
         if (!active) break;
         const int G = w + g * NW;                      // global group
         const int col = G / (NV / 16), k = (G % (NV / 16)) * 16 + n;
-        const unsigned short *pl = planes + col * 2 * PL;
+        const unsigned *pl = planes + col * PL;
         V b[KS];
 #pragma unroll
-        for (int ks = 0; ks < KS; ++ks) {
-            const int kc = 32 * ks + 8 * q;            // first K row of this lane
-            const int plane = kc / R, r0 = kc % R;
-            const u16x8 raw = *reinterpret_cast<const u16x8 *>(pl + plane * PL + pp(k * R + r0));
+        for (int ks = 0; ks < KS; ++ks) {              // K rows 32 ks + 8 q .. +7 = 4 complex samples
+            const u32x4 raw = *reinterpret_cast<const u32x4 *>(pl + pp(k * R + 16 * ks + 4 * q));
             b[ks] = __builtin_bit_cast(V, raw);
         }
 #pragma unroll
@@ -266,7 +275,7 @@ __device__ __forceinline__ void stage(unsigned short *planes, const pfa32::Tw &t
         const int col = G / (NV / 16), k = (G % (NV / 16)) * 16 + n;
         const int j = k % LS;
         const int obase = (k / LS) * LS * R + j;
-        unsigned short *pl = planes + col * 2 * PL;
+        unsigned *pl = planes + col * PL;
 #pragma unroll
         for (int mt = 0; mt < MT / 2; ++mt) {
 #pragma unroll
@@ -287,9 +296,7 @@ __device__ __forceinline__ void stage(unsigned short *planes, const pfa32::Tw &t
                         if (INV) wv.y = -wv.y;
                         y = pfa32::cmul(y, wv);
                     }
-                    const int p = pp(k2 * R2 + r2);
-                    pl[p] = to16<BF>(y.x);
-                    pl[PL + p] = to16<BF>(y.y);
+                    pl[pp(k2 * R2 + r2)] = pack<BF>(y);
                 }
             }
         }
@@ -298,7 +305,7 @@ __device__ __forceinline__ void stage(unsigned short *planes, const pfa32::Tw &t
 }
 
 template <int L, bool BF, bool INV, bool SPLIT, int NCOL, int NW, int S, typename Ops, typename Sink>
-__device__ __forceinline__ void stages_from(unsigned short *planes, const pfa32::Tw &tw,
+__device__ __forceinline__ void stages_from(unsigned *planes, const pfa32::Tw &tw,
                                             const Ops &ops, Sink &sink) {
     if constexpr (S < MPlan<L>::NS) {
         stage<L, BF, INV, SPLIT, NCOL, NW, S>(planes, tw, ops, sink);
@@ -308,17 +315,14 @@ __device__ __forceinline__ void stages_from(unsigned short *planes, const pfa32:
 
 // Store element e of line col (first-stage layout), already scaled.
 template <int L, bool BF>
-__device__ __forceinline__ void put(unsigned short *planes, int col, int e, float2 v) {
-    constexpr int PL = plane_len<L>();
-    const int p = pp(pos_of<L, 0>(e));
-    planes[col * 2 * PL + p] = to16<BF>(v.x);
-    planes[col * 2 * PL + PL + p] = to16<BF>(v.y);
+__device__ __forceinline__ void put(unsigned *planes, int col, int e, float2 v) {
+    planes[col * line_words<L>() + pp(pos_of<L, 0>(e))] = pack<BF>(v);
 }
 
 // Transform NCOL lines whose first-stage layout is in `planes` (after a
 // barrier); sink(col, index, value) receives DFT/N (or IDFT/N) in FP32.
 template <int L, bool BF, bool INV, bool SPLIT, int NCOL, int NW, typename Ops, typename Sink>
-__device__ __forceinline__ void fft(unsigned short *planes, const pfa32::Tw &tw, const Ops &ops,
+__device__ __forceinline__ void fft(unsigned *planes, const pfa32::Tw &tw, const Ops &ops,
                                     Sink &&sink) {
     static_assert(supported<L>(), "MFMA FFT sizes: 256, 512, 1024, 4096, 8192");
     stages_from<L, BF, INV, SPLIT, NCOL, NW, 0>(planes, tw, ops, sink);

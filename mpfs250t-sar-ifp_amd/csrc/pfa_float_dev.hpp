@@ -581,7 +581,7 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
                 for (int i = 0; i < NB / 64; ++i) lm = fmaxf(lm, wmax[i]);
                 const int se = pfam::norm_exp(lm);
                 const float sc = ldexpf(1.0f, -se);
-                unsigned short *planes = reinterpret_cast<unsigned short *>(reg);
+                unsigned *planes = reinterpret_cast<unsigned *>(reg);
 #pragma unroll
                 for (int g = 0; g < G; ++g) {
 #pragma unroll
@@ -599,8 +599,7 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
                 unsigned *dst = A16 + (size_t)j * M2;
                 pfam::fft<L, BF, INV, SPLIT, 1, NW>(planes, tw, pfam::OpsLds<L, BF, INV>{optab},
                     [&](int, int o, float2 y) {
-                        const unsigned pk = (unsigned)pfam::to16<BF>(y.x) | ((unsigned)pfam::to16<BF>(y.y) << 16);
-                        __builtin_nontemporal_store(pk, dst + o);
+                        __builtin_nontemporal_store(pfam::pack<BF>(y), dst + o);
                     });
                 if (t == 0) expA[j] = se + (INV ? 0 : L);
                 continue;

@@ -29,8 +29,8 @@ __global__ __launch_bounds__(fftb_mfma_nb<L>()) void k_fft_b_mfma(
     constexpr bool SPLIT = FK == FK_BF16S;
     constexpr int N2 = 1 << L, NCOL = 4, NB = fftb_mfma_nb<L>(), BW = NB / 64;
     constexpr int NW = pfam::nw_for<L, NCOL, BW>();
-    constexpr int PL = pfam::plane_len<L>();
-    __shared__ unsigned short planes[NCOL * 2 * PL];
+    constexpr int PL = pfam::line_words<L>();
+    __shared__ unsigned planes[NCOL * PL];
     __shared__ float2 twl[N2 / 64 + 64];
     __shared__ int emx[BW];
     const pfa32::Tw tw = pfa32::load_tw<N2>(twl, twg);
@@ -57,9 +57,8 @@ __global__ __launch_bounds__(fftb_mfma_nb<L>()) void k_fft_b_mfma(
         const unsigned wv[4] = {a.x, a.y, a.z, a.w};
 #pragma unroll
         for (int c = 0; c < NCOL; ++c) {
-            const float re = pfam::from16<BF>((unsigned short)(wv[c] & 0xffffu));
-            const float im = pfam::from16<BF>((unsigned short)(wv[c] >> 16));
-            pfam::put<L, BF>(planes, c, jj, make_float2(re * sc, im * sc));
+            const float2 v = pfam::unpack<BF>(wv[c]);
+            pfam::put<L, BF>(planes, c, jj, make_float2(v.x * sc, v.y * sc));
         }
     }
     __syncthreads();
