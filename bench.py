@@ -176,6 +176,68 @@ def precision_forms(spec, sc, geom, interp, forms, steps, warmup, torch):
     return out
 
 
+def autofocus_c5(interp, steps_af, torch):
+    """Gradient autofocus at 4096 x 4096 (64 targets, quadratic 3 rad + random-walk
+    phase error): ms of one sharpness forward, of one gradient step (forward +
+    adjoint), of the full focus of the same frame, and a steps_af-step Adam run."""
+    from mi355x_pfa import synth
+    from mi355x_pfa.autofocus import AutofocusPlan, autofocus
+    from mi355x_pfa.pfa import FloatPlan
+    spec = synth.C5
+    m, n = spec.pulses, spec.samples
+    sc, ax, ay, geom = make_inputs(spec, "cuda")
+    af = AutofocusPlan(sc.sig, geom, f0=sc.sc0, df=sc.scss, sgn=-1, interp=interp)
+    phi = torch.zeros(m, dtype=torch.float32, device="cuda")
+
+    def timed(fn, reps=10):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps * 1e3
+
+    def fwd():
+        return af.forward(phi)
+
+    def step():
+        S, img, st = af.forward(phi)
+        return af.backward(phi, img, st)
+
+    full = FloatPlan(m, n, -1, geom, f0=sc.sc0, df=sc.scss, window=True, interp=interp)
+    out = torch.empty((full.m2, full.n2), dtype=torch.float32, device="cuda")
+    t_fwd, t_step = timed(fwd), timed(step)
+    t_full = timed(lambda: full.focus_dev(sc.sig, out, detect=True))
+    full.close()
+    s0 = float(af.forward(phi)[0])
+    t0 = time.perf_counter()
+    phi_hat, hist = autofocus(af, steps=steps_af, lr=0.1)
+    torch.cuda.synchronize()
+    t_af = time.perf_counter() - t0
+    s1 = float(af.forward(phi_hat)[0])
+    t = np.arange(m)
+
+    def detrend(v):
+        return v - np.polyval(np.polyfit(t, v, 1), t)
+
+    inj = detrend(sc.phase_error)
+    res = detrend(phi_hat.cpu().numpy().astype(np.float64) + sc.phase_error)
+    af.close()
+    return {"workload": f"C5: {m} pulses x {n} samples -> 4096^2, {interp}, FP32, "
+                        "quadratic 3 rad + random walk 0.05 rad/pulse phase error",
+            "ms_sharpness_forward": round(t_fwd, 3),
+            "ms_gradient_step": round(t_step, 3),
+            "ms_focus_full": round(t_full, 3),
+            "gradient_step_over_forward": round(t_step / t_fwd, 2),
+            "gradient_step_over_full_focus": round(t_step / t_full, 2),
+            "steps": steps_af, "s_autofocus_total": round(t_af, 2),
+            "sharpness_before_after": [float(f"{s0:.5e}"), float(f"{s1:.5e}")],
+            "phase_error_rms_rad_before_after": [round(float(inj.std()), 4),
+                                                 round(float(res.std()), 4)]}
+
+
 def cpu_baseline(spec, interp, geom, budget_rows=400):
     """The numpy oracle on the host, on a bounded sample of the frame: pass 1 on
     `budget_rows` pulses and pass 2 on `budget_rows` range bins of the real
@@ -233,6 +295,8 @@ def main():
                     help="FFT form of the timed frames (value)")
     ap.add_argument("--forms", default="fp32,bf16,fp16",
                     help="FFT forms timed beside, with their error against FP64 ('' = none)")
+    ap.add_argument("--autofocus-steps", type=int, default=50,
+                    help="gradient autofocus steps at 4096^2 (0 = skip)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: the focus path is built and timed on a single "
@@ -326,6 +390,9 @@ def main():
         print("[bench] FFT forms vs FP64", file=sys.stderr, flush=True)
         res["fft_forms"] = precision_forms(spec, sc, geom, a.interp, forms, a.steps, a.warmup,
                                            torch)
+    if a.autofocus_steps > 0:
+        print("[bench] gradient autofocus 4096^2", file=sys.stderr, flush=True)
+        res["autofocus"] = autofocus_c5(a.interp, a.autofocus_steps, torch)
     if not a.no_fixed and a.config == "C3":
         print("[bench] fixed-point frames", file=sys.stderr, flush=True)
         fwall, fk = time_fixed(sc, ax, ay, geom, max(3, a.steps // 2), 2, torch)

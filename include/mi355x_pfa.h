@@ -188,6 +188,25 @@ int pfa_resample_kspace_f32(pfa_plan *plan, const float *sig_c64, const double *
 /* focus (form_image_pfa.py:225-240) of a given g2 complex64 [m][n]. */
 int pfa_transform_f32(pfa_plan *plan, const float *g2_c64, void *out, int out_kind);
 
+/* ------------------------------------------------------------------ autofocus
+ * Gradient autofocus on the float path.  No counterpart in the reference: for
+ * callers that focus with per-pulse phase corrections phi[m] (original pulse
+ * order, radians): sig'[i][k] = sig[i][k] e^{j phi[i]}, I = pfa(sig') (complex,
+ * [m2][n2]), sharpness S = sum |I|^4 / (sum |I|^2)^2.  FP32 FFT form only;
+ * padded cross size m2 <= 4096.  All pointers are device pointers; work is
+ * enqueued on the context stream.
+ *   pfa_af_checkpoint  range pass of sig once (the phase commutes with it); its
+ *                      corner-turned output stays in the plan's workspace
+ *   pfa_af_forward     img[m2][n2] complex64 of the corrected history and
+ *                      stats[3] = {sum |I|^2, sum |I|^4, S} (double)
+ *   pfa_af_backward    grad[m] = dS/dphi from img and stats of that forward,
+ *                      by the adjoints of the azimuth pass, window and both
+ *                      FFT passes (the transpose gather as a segmented sum) */
+int pfa_af_checkpoint(pfa_plan *plan, const float *sig_c64, const double *freq);
+int pfa_af_forward(pfa_plan *plan, const float *phi, float *img_c64, double *stats);
+int pfa_af_backward(pfa_plan *plan, const float *phi, const float *img_c64,
+                    const double *stats, float *grad);
+
 void pfa_plan_destroy(pfa_plan *plan);
 
 #ifdef __cplusplus

@@ -636,6 +636,74 @@ extern "C" int pfa_focus_f32_dev(pfa_plan *p, const float *sig, const double *fr
     return PFA_OK;
 }
 
+// ------------------------------------------------------------------ autofocus
+static int af_check(pfa_plan *p) {
+    if (!p || p->kind != PLAN_FLOAT) return PFA_ERR_STATE;
+    if (p->flags & (PFA_FLOAT_FFT_BF16 | PFA_FLOAT_FFT_FP16))
+        return pfa_fail(p->ctx, PFA_ERR_ARG, "autofocus: needs the FP32 FFT form");
+    if (p->m2 > 4096) return pfa_fail(p->ctx, PFA_ERR_DIMS, "autofocus: padded cross size above 4096");
+    return PFA_OK;
+}
+
+extern "C" int pfa_af_checkpoint(pfa_plan *p, const float *sig, const double *freq) {
+    CK(af_check(p));
+    if (!sig) return pfa_fail(p->ctx, PFA_ERR_ARG, "af_checkpoint: null signal");
+    if (!freq && !p->f0d) return pfa_fail(p->ctx, PFA_ERR_ARG, "af_checkpoint: plan has no compact frequencies; pass freq");
+    int st = PFA_OK;
+    const int ldp = ldp32(p->m);
+    void *g1t = pfa_ws(p->ctx, "af_g1t", (size_t)p->n * ldp * 8, &st);
+    if (!g1t) return st;
+    CK(launch_float_range(p, sig, freq, g1t, ldp));
+    p->af_ready = true;
+    return PFA_OK;
+}
+
+static int af_buffers(pfa_plan *p, void **g1t, void **rot, const float *phi) {
+    int st = PFA_OK;
+    *g1t = pfa_ws(p->ctx, "af_g1t", (size_t)p->n * ldp32(p->m) * 8, &st);
+    if (!*g1t) return st;
+    *rot = pfa_ws(p->ctx, "af_rot", (size_t)p->m * 8, &st);
+    if (!*rot) return st;
+    return launch_af_rot(p, phi, *rot);
+}
+
+extern "C" int pfa_af_forward(pfa_plan *p, const float *phi, float *img, double *stats) {
+    CK(af_check(p));
+    if (!p->af_ready) return pfa_fail(p->ctx, PFA_ERR_STATE, "af_forward: no checkpoint (call pfa_af_checkpoint)");
+    if (!phi || !img || !stats) return pfa_fail(p->ctx, PFA_ERR_ARG, "af_forward: null pointer");
+    int st = PFA_OK;
+    void *g1t, *rot;
+    CK(af_buffers(p, &g1t, &rot, phi));
+    void *A = pfa_ws(p->ctx, "f_A", (size_t)(p->n + 15) / 16 * 16 * p->m2 * 8, &st);
+    if (!A) return st;
+    constexpr int parts = 1024;
+    double *part = (double *)pfa_ws(p->ctx, "af_stat", parts * 2 * sizeof(double), &st);
+    if (!part) return st;
+    CK(launch_float_azimuth(p, 1, g1t, ldp32(p->m), nullptr, A, rot));
+    CK(launch_float_fft_b(p, A, img, PFA_OUT_C64));
+    return launch_af_stats(p, img, part, parts, stats);
+}
+
+extern "C" int pfa_af_backward(pfa_plan *p, const float *phi, const float *img,
+                               const double *stats, float *grad) {
+    CK(af_check(p));
+    if (!p->af_ready) return pfa_fail(p->ctx, PFA_ERR_STATE, "af_backward: no checkpoint (call pfa_af_checkpoint)");
+    if (!phi || !img || !stats || !grad) return pfa_fail(p->ctx, PFA_ERR_ARG, "af_backward: null pointer");
+    int st = PFA_OK;
+    void *g1t, *rot;
+    CK(af_buffers(p, &g1t, &rot, phi));
+    void *B = pfa_ws(p->ctx, "af_B", (size_t)p->m2 * p->n * 8, &st);
+    if (!B) return st;
+    const size_t max_parts = (size_t)std::max(p->ctx->num_cu, 8) + 8;   // >= the adjoint's grid
+    float *part = (float *)pfa_ws(p->ctx, "af_part", max_parts * p->m * 4, &st);
+    if (!part) return st;
+    CK(launch_af_fftb_adj(p, img, stats, B));
+    int nparts = 0;
+    CK(launch_af_az_adj(p, g1t, ldp32(p->m), rot, B, part, &nparts));
+    if ((size_t)nparts > max_parts) return pfa_fail(p->ctx, PFA_ERR_STATE, "af_backward: partial buffer too small");
+    return launch_af_grad_reduce(p, part, nparts, stats, grad);
+}
+
 extern "C" int pfa_focus_f32(pfa_plan *p, const float *sig, const double *freq, void *out,
                              int out_kind) {
     CK(float_check(p, sig, freq, out, out_kind));

@@ -42,7 +42,7 @@ int launch_float_range(pfa_plan *p, const void *sig, const double *freq, void *g
     const bool sinc = p->flags & PFA_FLOAT_SINC32;
     const bool full = freq != nullptr;
     pfa_timer_begin(ctx, PFA_TIMER_RANGE);
-#define ARGS (const float2 *)sig, p->m, p->n, p->order, p->a64, p->f0d, p->dfd, freq, grid_of(p->KR, p->kr_lin, p->n), p->sinc, \
+#define ARGS (const float2 *)sig, p->m, p->n, p->order, p->a64, p->pr64, p->f0d, p->dfd, freq, grid_of(p->KR, p->kr_lin, p->n), p->sinc, \
              (float2 *)g1t, ldp, (unsigned *)(ctx->d_counters + 32)
     if (sinc && full) hipLaunchKernelGGL((k_range_f32<true, true>), grid, dim3(RT), 0, ctx->stream, ARGS);
     else if (sinc) hipLaunchKernelGGL((k_range_f32<true, false>), grid, dim3(RT), 0, ctx->stream, ARGS);
@@ -55,7 +55,7 @@ int launch_float_range(pfa_plan *p, const void *sig, const double *freq, void *g
 }
 
 int launch_float_azimuth(pfa_plan *p, int mode, const void *g1t, int ldp, const void *g2in,
-                         void *out) {
+                         void *out, const void *rot) {
     pfa_ctx *ctx = p->ctx;
     int st = PFA_OK;
     const int m2 = p->m2;
@@ -69,7 +69,7 @@ int launch_float_azimuth(pfa_plan *p, int mode, const void *g1t, int ldp, const 
     pfa_timer_begin(ctx, PFA_TIMER_AZIMUTH);
     const TanBins tb{p->tan_bin, p->tb_lo, p->tb_inv_w, p->tb_bins};
 #define ARGS (const float2 *)g1t, ldp, p->m, p->n, grid_of(p->KR, p->kr_lin, p->n), p->tan_s64, tb, grid_of(p->KC, p->kc_lin, p->m), p->hm, p->hn, p->sinc, \
-             (const float2 *)g2in, (float2 *)out, tw, (unsigned *)(ctx->d_counters + 32), nullptr, nullptr
+             (const float2 *)g2in, (float2 *)out, tw, (unsigned *)(ctx->d_counters + 32), nullptr, nullptr, (const float2 *)rot
 #define K4(LL, MODE)                                                                                 \
     if (sinc && inv) hipLaunchKernelGGL((k_azimuth_f32<LL, MODE, true, true>), dim3(grid_az), dim3(az_nb<LL>()), 0, ctx->stream, ARGS); \
     else if (sinc) hipLaunchKernelGGL((k_azimuth_f32<LL, MODE, true, false>), dim3(grid_az), dim3(az_nb<LL>()), 0, ctx->stream, ARGS); \

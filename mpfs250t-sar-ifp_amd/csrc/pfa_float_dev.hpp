@@ -166,8 +166,8 @@ __device__ __forceinline__ int bracket_from(XA &xa, int S, double Q, int g, doub
 template <bool SINC, bool FULLFREQ>
 __global__ __launch_bounds__(RT, 2) void k_range_f32(
     const float2 *__restrict__ sig, int m, int n, const int32_t *__restrict__ order,
-    const double *__restrict__ a64, const double *__restrict__ f0v, const double *__restrict__ dfv,
-    const double *__restrict__ freq, const Grid1 KR,
+    const double *__restrict__ a64, const double *__restrict__ pr64, const double *__restrict__ f0v,
+    const double *__restrict__ dfv, const double *__restrict__ freq, const Grid1 KR,
     const int16_t *__restrict__ sinc, float2 *__restrict__ g1t, int ldp,
     unsigned *__restrict__ nslow) {
     static_assert(sizeof(float2) * RQ * (RP + 1) <= sizeof(float2) * RP * RSP, "tile fits the strips");
@@ -202,17 +202,27 @@ __global__ __launch_bounds__(RT, 2) void k_range_f32(
             }
         }
         const double *frow = freq + (size_t)i * n;
-        auto kr = [&](int j) {
-            const double f = FULLFREQ ? frow[j] : __dadd_rn(f0, __dmul_rn((double)j, df));
-            return __dmul_rn(ai, f);
-        };
-        const bool asc = p < m ? !(kr(0) > kr(n - 1)) : true;
+        auto fq = [&](int j) { return FULLFREQ ? frow[j] : __dadd_rn(f0, __dmul_rn((double)j, df)); };
+        // kr as (2 pr / c) f: one product per position, within a few ulp of numpy's
+        // (2 f / c) pr; kr_np is numpy's rounding exactly, used to decide near-ties
+        // (the KR grid's end points are such ties by construction)
+        auto kr = [&](int j) { return __dmul_rn(ai, fq(j)); };
+        const double prv = p < m ? pr64[i] : 1.0;
+        auto kr_np = [&](int j) { return __dmul_rn(__ddiv_rn(2.0 * fq(j), C_LIGHT), prv); };
+        const bool asc = p < m ? !(kr_np(0) > kr_np(n - 1)) : true;
         auto xa = [&](int u) { return kr(asc ? u : n - 1 - u); };
+        auto xe = [&](int u) { return kr_np(asc ? u : n - 1 - u); };
         const double x0 = ai * f0, inv_adf = 1.0 / (ai * df);
         auto guess = [&](double Q) {                  // ascending-view count estimate
             const double js = (Q - x0) * inv_adf;
             const double g = asc ? js : (double)(n - 1) - js;
             return (g == g) ? (int)fmin(fmax(floor(g) + 1.0, 0.0), (double)n) : 0;
+        };
+        auto bracket = [&](double Q, double &xl, double &xh) {
+            int c = bracket_from(xa, n, Q, guess(Q), xl, xh);
+            const double tol = fabs(Q) * 4e-15;
+            if (fabs(xl - Q) <= tol || fabs(xh - Q) <= tol) c = bracket_from(xe, n, Q, c, xl, xh);
+            return c;
         };
         const float2 *row = sig + (size_t)i * n;
         auto fa_g = [&](int u) { return row[asc ? u : n - 1 - u]; };
@@ -222,7 +232,7 @@ __global__ __launch_bounds__(RT, 2) void k_range_f32(
             if (p < m) {
                 double xl, xh;
                 const double Q = KR(q0);
-                const int c = bracket_from(xa, n, Q, guess(Q), xl, xh);
+                const int c = bracket(Q, xl, xh);
                 lo = max(0, min(c - 1 - 15 - 2, n - RS));
                 hi = min(lo + RS, n);
             }
@@ -256,7 +266,7 @@ __global__ __launch_bounds__(RT, 2) void k_range_f32(
                 if (p >= m || q >= qe) continue;
                 const double Q = KR(q);
                 double xl, xh;
-                const int c = bracket_from(xa, n, Q, guess(Q), xl, xh);
+                const int c = bracket(Q, xl, xh);
                 int ph = 0;
                 if (c == n) {
                     ph = (Q == xl) ? 1 : 0;           // last sample, or outside
@@ -392,7 +402,7 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
     const double *__restrict__ hm, const double *__restrict__ hn,
     const int16_t *__restrict__ sinc, const float2 *__restrict__ g2in,
     float2 *__restrict__ out, const float2 *__restrict__ twg, unsigned *__restrict__ nslow,
-    unsigned *__restrict__ A16, int *__restrict__ expA) {
+    unsigned *__restrict__ A16, int *__restrict__ expA, const float2 *__restrict__ rot) {
     static_assert(FK == FK_F32 || MODE == 1, "matrix-core FFT only in the fused mode");
     constexpr bool BF = FK == FK_BF16 || FK == FK_BF16S;
     constexpr bool SPLIT = FK == FK_BF16S;
@@ -420,12 +430,14 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
     auto active_at = [&](int base) { return base + rank < per_x && line_of(base) < n; };
     unsigned slow = 0;
     float2 pre[PR];                                    // next row, in flight
+    // rot (optional): per sorted pulse e^{j phi} of a phase correction (autofocus)
     auto prefetch = [&](int j) {
         const float2 *row = g1t + (size_t)j * ldp;
 #pragma unroll
         for (int k = 0; k < PR; ++k) {
             const int p = t + k * NB;
             pre[k] = p < m ? ld_stream(row + p) : make_float2(0.f, 0.f);
+            if (rot && p < m) pre[k] = pfa32::cmul(pre[k], rot[p]);
         }
     };
     if (RES && active_at(0)) prefetch(line_of(0));

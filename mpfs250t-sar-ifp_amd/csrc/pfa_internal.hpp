@@ -57,6 +57,7 @@ struct pfa_plan {
     // bracket guesses of the azimuth pass: tan_bin[b] = #{p : tan_s[p] < tb_lo + b / tb_inv_w},
     // b = 0..tb_bins (a starting point only; the kernels fix brackets up exactly)
     int32_t *tan_bin = nullptr;
+    bool af_ready = false;                // autofocus checkpoint (range pass) computed
     int tb_bins = 0;
     double tb_lo = 0.0, tb_inv_w = 0.0;
     std::vector<void *> owned;            // device allocations of the plan
@@ -103,8 +104,17 @@ int launch_fixed_detect_t(pfa_plan *p, const void *yb, const int32_t *eb, const 
 
 // ---------------------------------------------------------------- float launchers
 int launch_float_range(pfa_plan *p, const void *sig, const double *freq, void *g1t, int ldp);
+// rot: optional per sorted pulse e^{j phi} applied to the pass-1 rows (autofocus)
 int launch_float_azimuth(pfa_plan *p, int mode, const void *g1t, int ldp, const void *g2in,
-                         void *out);
+                         void *out, const void *rot = nullptr);
+// gradient autofocus (pfa_af.hip)
+int launch_af_rot(pfa_plan *p, const float *phi, void *rot);
+int launch_af_stats(pfa_plan *p, const void *img, double *partial, int nparts, double *stats);
+int launch_af_fftb_adj(pfa_plan *p, const void *img, const double *stats, void *B);
+int launch_af_az_adj(pfa_plan *p, const void *g1t, int ldp, const void *rot, const void *B,
+                     float *partial, int *nparts);
+int launch_af_grad_reduce(pfa_plan *p, const float *partial, int nparts, const double *stats,
+                          float *grad);
 int launch_float_fft_b(pfa_plan *p, const void *A, void *out, int out_kind);
 // matrix-core FFT form; fk: 1 BF16, 2 FP16, 3 BF16 with hi/lo DFT matrices
 int launch_float_azimuth_mfma(pfa_plan *p, const void *g1t, int ldp, unsigned *A16, int *expA,

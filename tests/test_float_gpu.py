@@ -109,3 +109,16 @@ def test_compact_frequencies_equal_full_array(gp):
     a = p_full.focus(sc.sig, freq)
     b = p_c.focus(sc.sig)
     assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("m,n,interp", [(100, 90, "linear"), (64, 90, "linear"), (100, 100, "sinc32"),
+                                        (128, 90, "linear")])
+def test_resample_kspace_elementwise_against_oracle(gp, m, n, interp):
+    """Every k-space sample, not only the image RMS: the range grid's end points
+    coincide with some pulse's own k-space samples, and np.interp's decision
+    there (last sample vs zero) follows numpy's rounding of kr = (2 f / c) pr."""
+    sc, freq, ax, ay = _scene(m, n, seed=m + n)
+    print(f"resample_kspace elementwise {m}x{n} {interp}", flush=True)
+    g2o, _ = rfl.resample_kspace(sc.sig, freq, ax, ay, interp=interp)
+    g2g, _ = gp.resample_kspace(sc.sig, freq, ax, ay, interp=interp)
+    assert np.abs(g2g.astype(np.complex128) - g2o).max() <= 1e-5 * np.abs(g2o).max()
