@@ -10,6 +10,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace pfa32 {
 
 constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v >> 1); }
@@ -128,6 +130,15 @@ __device__ __forceinline__ Tw load_tw(float2 *lds, const float2 *__restrict__ g)
     return Tw{lds, lds + N / 64};
 }
 
+// A first-pass source is called as src(e) or, when it takes two arguments, as
+// src(e, s) with the compile-time slot s: e = threadIdx.x + s NT, so a source
+// holding a thread's elements in a register array can index it statically.
+template <typename Src>
+__device__ __forceinline__ float2 call_src(Src &src, int e, int s) {
+    if constexpr (std::is_invocable_v<Src &, int, int>) return src(e, s);
+    else return src(e);
+}
+
 // One Stockham pass over the LDS line `buf`; LS = product of earlier radices.
 // On the last pass the outputs go to sink(index, value) instead of LDS.
 template <int L, int EMAX, int R, int LS, bool INV, bool LAST, typename Src, typename Sink>
@@ -145,7 +156,7 @@ __device__ __forceinline__ void pass(float2 *buf, const Tw &tw, Src *src, Sink &
             const int k = t + b * NT;
 #pragma unroll
             for (int r = 0; r < R; ++r)
-                x[b][r] = src ? (*src)(k + r * (N / R)) : buf[padf(k + r * (N / R))];
+                x[b][r] = src ? call_src(*src, k + r * (N / R), b + r * B) : buf[padf(k + r * (N / R))];
         }
     }
     if (!src) __syncthreads();

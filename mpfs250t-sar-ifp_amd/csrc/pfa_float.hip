@@ -102,10 +102,10 @@ int launch_float_fft_b(pfa_plan *p, const void *A, void *out, int out_kind) {
 #define ARGS (const float2 *)A, p->n, p->m2, out, tw
 #define LAUNCH(LL)                                                                                   \
     case LL:                                                                                         \
-        if (inv && out_kind) hipLaunchKernelGGL((k_fft_b_f32<LL, true, 1>), dim3(p->m2), dim3(nb_of<LL>()), 0, ctx->stream, ARGS); \
-        else if (inv) hipLaunchKernelGGL((k_fft_b_f32<LL, true, 0>), dim3(p->m2), dim3(nb_of<LL>()), 0, ctx->stream, ARGS); \
-        else if (out_kind) hipLaunchKernelGGL((k_fft_b_f32<LL, false, 1>), dim3(p->m2), dim3(nb_of<LL>()), 0, ctx->stream, ARGS); \
-        else hipLaunchKernelGGL((k_fft_b_f32<LL, false, 0>), dim3(p->m2), dim3(nb_of<LL>()), 0, ctx->stream, ARGS); \
+        if (inv && out_kind) hipLaunchKernelGGL((k_fft_b2_f32<LL, true, 1>), dim3(p->m2 / 2), dim3(nb_of<LL>()), 0, ctx->stream, ARGS); \
+        else if (inv) hipLaunchKernelGGL((k_fft_b2_f32<LL, true, 0>), dim3(p->m2 / 2), dim3(nb_of<LL>()), 0, ctx->stream, ARGS); \
+        else if (out_kind) hipLaunchKernelGGL((k_fft_b2_f32<LL, false, 1>), dim3(p->m2 / 2), dim3(nb_of<LL>()), 0, ctx->stream, ARGS); \
+        else hipLaunchKernelGGL((k_fft_b2_f32<LL, false, 0>), dim3(p->m2 / 2), dim3(nb_of<LL>()), 0, ctx->stream, ARGS); \
         break;
     PFA_F32_SWITCH(L, LAUNCH)
 #undef LAUNCH

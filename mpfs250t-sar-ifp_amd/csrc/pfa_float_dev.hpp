@@ -644,27 +644,45 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
 }
 
 // ------------------------------------------------------------------ FFT B (range) + detect
-// OUT 0: complex64 image, 1: |image| float32.  Workgroup b -> cross bin r with
-// consecutive bins on the same XCD.
+// OUT 0: complex64 image, 1: |image| float32, image [cross r][range c]; the
+// column loads of A are the corner-turn.  Consecutive workgroups on one XCD.
+// Two adjacent cross bins r0, r0 + 1 per workgroup: every row of A is read as
+// one 16-byte load (half the memory requests of one column per workgroup).
+// Thread t holds elements t + s NT (s < E) of both columns -- exactly the
+// inputs of its first Stockham pass -- transforms the first column through
+// the LDS line, then the second from its registers through the same line.
 template <int L, bool INV, int OUT>
-__global__ __launch_bounds__(pfa32::Cfg<L>::NT < 64 ? 64 : pfa32::Cfg<L>::NT) void k_fft_b_f32(
+__global__ __launch_bounds__(pfa32::Cfg<L>::NT < 64 ? 64 : pfa32::Cfg<L>::NT, 2) void k_fft_b2_f32(
     const float2 *__restrict__ A, int n, int m2, void *__restrict__ outp,
     const float2 *__restrict__ twg) {
     using Cf = pfa32::Cfg<L>;
-    constexpr int N2 = Cf::N;
+    constexpr int N2 = Cf::N, NT = Cf::NT, E = Cf::E;
     __shared__ float2 buf[Cf::PADN];
     __shared__ float2 twl[N2 / 64 + 64];
     const pfa32::Tw tw = pfa32::load_tw<N2>(twl, twg);
-    const int r = xcd_map(blockIdx.x, m2);
+    const int r0 = 2 * xcd_map(blockIdx.x, m2 / 2);
+    const int t = threadIdx.x;
     const float scale = INV ? 1.0f / (float)N2 : 1.0f;
-    // the first pass reads its butterflies straight from A (the corner-turn)
-    auto src = [&](int jj) { return jj < n ? A[ablk(jj, r, m2)] : make_float2(0.f, 0.f); };
-    pfa32::fft_src<L, INV>(buf, tw, src, [&](int c, float2 v) {
-        if (INV) v = make_float2(v.x * scale, v.y * scale);
-        if ((r + c) & 1) v = make_float2(-v.x, -v.y);
-        if (OUT == 0) reinterpret_cast<float2 *>(outp)[(size_t)r * N2 + c] = v;
-        else reinterpret_cast<float *>(outp)[(size_t)r * N2 + c] = __fsqrt_rn(v.x * v.x + v.y * v.y);
-    });
+    float2 c0[E], c1[E];
+#pragma unroll
+    for (int s2 = 0; s2 < E; ++s2) {
+        const int jj = t + s2 * NT;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (t < NT && jj < n) v = *reinterpret_cast<const float4 *>(A + ablk(jj, r0, m2));
+        c0[s2] = make_float2(v.x, v.y);
+        c1[s2] = make_float2(v.z, v.w);
+    }
+    // the first pass asks for element t + s NT as slot s (pass 0 has LS = 1)
+    auto sink_for = [&](int r) {
+        return [&, r](int c, float2 v) {
+            if (INV) v = make_float2(v.x * scale, v.y * scale);
+            if ((r + c) & 1) v = make_float2(-v.x, -v.y);
+            if (OUT == 0) reinterpret_cast<float2 *>(outp)[(size_t)r * N2 + c] = v;
+            else reinterpret_cast<float *>(outp)[(size_t)r * N2 + c] = __fsqrt_rn(v.x * v.x + v.y * v.y);
+        };
+    };
+    pfa32::fft_src<L, INV>(buf, tw, [&](int, int s2) { return c0[s2]; }, sink_for(r0));
+    pfa32::fft_src<L, INV>(buf, tw, [&](int, int s2) { return c1[s2]; }, sink_for(r0 + 1));
 }
 
 // Two-level twiddle tables of an n-point line: hi[k] = W^(64 k), k < n/64,
