@@ -194,6 +194,21 @@ struct OpsLds {
     }
 };
 
+// The same table in global memory (built once per size and precision by
+// k_build_ops, read through the caches): 16 bytes per lane and tile.
+template <int L, bool BF, bool INV>
+struct OpsGlobal {
+    const u16x8 *__restrict__ tab;
+    template <int S>
+    __device__ __forceinline__ void get(int mt, int ks, int lane, typename H16<BF>::V &hi,
+                                        typename H16<BF>::V &lo) const {
+        constexpr int KS = 2 * MPlan<L>::R[S] / 32;
+        const int t = tile0<L, S>() + mt * KS + ks;
+        hi = __builtin_bit_cast(typename H16<BF>::V, tab[t * 64 + lane]);
+        lo = __builtin_bit_cast(typename H16<BF>::V, tab[(ntiles<L>() + t) * 64 + lane]);
+    }
+};
+
 template <int L, bool BF, bool INV, int S>
 __device__ __forceinline__ void build_ops_stage(u16x8 *tab) {
     if constexpr (S < MPlan<L>::NS) {

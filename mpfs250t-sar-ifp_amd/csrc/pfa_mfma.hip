@@ -20,11 +20,17 @@ namespace {
 template <int L>
 constexpr int fftb_mfma_nb() { return 64 * pfam::nw_for<L, 4, 8>(); }
 
+// DFT-matrix operand table of an L-point matrix-core FFT (pfam::build_ops layout).
+template <int L, bool BF, bool INV>
+__global__ __launch_bounds__(256) void k_build_ops(pfam::u16x8 *tab) {
+    pfam::build_ops_stage<L, BF, INV, 0>(tab);
+}
+
 // OUT 0: complex64 image, 1: |image| float32; image [cross r][range c].
 template <int L, bool INV, int OUT, int FK>
 __global__ __launch_bounds__(fftb_mfma_nb<L>()) void k_fft_b_mfma(
     const unsigned *__restrict__ A16, const int *__restrict__ expA, int n, int m2,
-    void *__restrict__ outp, const float2 *__restrict__ twg) {
+    void *__restrict__ outp, const float2 *__restrict__ twg, const pfam::u16x8 *__restrict__ ops) {
     constexpr bool BF = FK == FK_BF16 || FK == FK_BF16S;
     constexpr bool SPLIT = FK == FK_BF16S;
     constexpr int N2 = 1 << L, NCOL = 4, NB = fftb_mfma_nb<L>(), BW = NB / 64;
@@ -63,7 +69,7 @@ __global__ __launch_bounds__(fftb_mfma_nb<L>()) void k_fft_b_mfma(
     }
     __syncthreads();
     const float osc = ldexpf(1.0f, S + (INV ? 0 : L));
-    pfam::fft<L, BF, INV, SPLIT, NCOL, NW>(planes, tw, pfam::OpsReg<L, BF, INV>{},
+    pfam::fft<L, BF, INV, SPLIT, NCOL, NW>(planes, tw, pfam::OpsGlobal<L, BF, INV>{ops},
         [&](int col, int c, float2 y) {
             const int r = r0 + col;
             float2 v = make_float2(y.x * osc, y.y * osc);
@@ -119,6 +125,34 @@ int launch_float_azimuth_mfma(pfa_plan *p, const void *g1t, int ldp, unsigned *A
     return PFA_OK;
 }
 
+// The operand table of (L, precision, direction), built on first use and kept
+// in the context's workspace.
+static const pfam::u16x8 *dft_ops(pfa_ctx *ctx, int L, bool bf, bool inv, int *st) {
+    char slot[48];
+    snprintf(slot, sizeof slot, "dftops_%d_%d_%d", L, (int)bf, (int)inv);
+    auto it = ctx->ws.find(slot);
+    if (it != ctx->ws.end() && it->second.bytes) return (const pfam::u16x8 *)it->second.ptr;
+    int nt = 0;
+    switch (L) {
+        case 8: nt = pfam::ntiles<8>(); break;
+        case 9: nt = pfam::ntiles<9>(); break;
+        case 10: nt = pfam::ntiles<10>(); break;
+        case 12: nt = pfam::ntiles<12>(); break;
+        case 13: nt = pfam::ntiles<13>(); break;
+        default: *st = pfa_fail(ctx, PFA_ERR_DIMS, "matrix-core FFT sizes are 256, 512, 1024, 4096 and 8192"); return nullptr;
+    }
+    auto *tab = (pfam::u16x8 *)pfa_ws(ctx, slot, (size_t)2 * nt * 64 * sizeof(pfam::u16x8), st);
+    if (!tab) return nullptr;
+#define B(LL)                                                                                              case LL:                                                                                                   if (bf && inv) hipLaunchKernelGGL((k_build_ops<LL, true, true>), dim3(1), dim3(256), 0, ctx->stream, tab);          else if (bf) hipLaunchKernelGGL((k_build_ops<LL, true, false>), dim3(1), dim3(256), 0, ctx->stream, tab);           else if (inv) hipLaunchKernelGGL((k_build_ops<LL, false, true>), dim3(1), dim3(256), 0, ctx->stream, tab);          else hipLaunchKernelGGL((k_build_ops<LL, false, false>), dim3(1), dim3(256), 0, ctx->stream, tab);                  break;
+    switch (L) { B(8) B(9) B(10) B(12) B(13) default: break; }
+#undef B
+    if (hipGetLastError() != hipSuccess) {
+        *st = pfa_fail(ctx, PFA_ERR_HIP, "DFT operand table build failed");
+        return nullptr;
+    }
+    return tab;
+}
+
 int launch_float_fft_b_mfma(pfa_plan *p, const unsigned *A16, const int *expA, void *out,
                             int out_kind, int fk) {
     pfa_ctx *ctx = p->ctx;
@@ -128,9 +162,11 @@ int launch_float_fft_b_mfma(pfa_plan *p, const unsigned *A16, const int *expA, v
     if (!tw) return st;
     const bool inv = p->sgn > 0;
     const int L = log2_exact(n2);
+    const pfam::u16x8 *ops = dft_ops(ctx, L, fk != FK_FP16, inv, &st);
+    if (!ops) return st;
     const dim3 grid(p->m2 / 4);
     pfa_timer_begin(ctx, PFA_TIMER_FFT_B);
-#define ARGS A16, expA, p->n, p->m2, out, tw
+#define ARGS A16, expA, p->n, p->m2, out, tw, ops
 #define K(LL, I, O, F) hipLaunchKernelGGL((k_fft_b_mfma<LL, I, O, F>), grid, dim3(fftb_mfma_nb<LL>()), 0, ctx->stream, ARGS)
 #define KF(LL, F)                              \
     if (inv && out_kind) K(LL, true, 1, F);    \
