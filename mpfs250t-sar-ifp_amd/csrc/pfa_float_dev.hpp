@@ -43,8 +43,8 @@ __device__ __forceinline__ void st_stream(float2 *p, float2 v) {     // write-on
     w.y = v.y;
     __builtin_nontemporal_store(w, reinterpret_cast<f2v *>(p));
 }
-// Taps of the Q15 sinc table as float (exact: c / 2^15), staged in LDS
-// tap-major: tap t of phase ph at tapl[(t + 4) * 256 + scr(ph)], with four zero
+// Taps of the Q15 sinc table as float (exact: c / 2^15; the autofocus adjoint's
+// table), staged in LDS tap-major: tap t of phase ph at tapl[(t + 4) * 256 + scr(ph)], with four zero
 // rows on either side (t in [-4, 36)) so a window shifted by up to KD samples
 // reads zeros instead of needing masks, and scr(ph) = ph ^ (ph >> 3) spreads
 // the phases of a wave's lanes over the LDS banks.
@@ -57,6 +57,25 @@ __device__ __forceinline__ void load_taps(float *tapl, const int16_t *__restrict
     }
 }
 
+// The same taps packed as Q15 integer pairs for the sinc gathers: word
+// tp[h * TPH + r * 256 + scr(ph)] holds rows (2r + h, 2r + h + 1) of the
+// zero-padded tap rows (low half first), h = 0 or 1, r < 20, so a window
+// shifted by any d reads aligned pairs.  Half the LDS bytes of float taps; one
+// conversion per tap; sums of integer taps scaled by 2^-15 at the end equal
+// the float-tap sums exactly (a power-of-two scale commutes with rounding).
+constexpr int TPR = TROWS / 2, TPH = TPR * 256, TPWORDS = 2 * TPH;
+__device__ __forceinline__ void load_taps2(unsigned *tp, const int16_t *__restrict__ sinc) {
+    for (int i = threadIdx.x; i < TPWORDS; i += blockDim.x) {
+        const int h = i / TPH, r = (i / 256) % TPR, ph = i & 255;
+        const int ra = 2 * r + h - 4, rb = ra + 1;           // tap indices of the two rows
+        const int a = (ra >= 0 && ra < 32) ? sinc[ph * 32 + ra] : 0;
+        const int b = (rb >= 0 && rb < 32) ? sinc[ph * 32 + rb] : 0;
+        tp[h * TPH + r * 256 + scr(ph)] = ((unsigned)a & 0xffffu) | ((unsigned)b << 16);
+    }
+}
+__device__ __forceinline__ float tap_lo(unsigned w) { return (float)(short)(w & 0xffffu); }
+__device__ __forceinline__ float tap_hi(unsigned w) { return (float)((int)w >> 16); }
+
 constexpr int KD = 4;            // bracket advance covered by one 4-output window
 constexpr int KW = 32 + KD;      // samples in that window
 
@@ -66,53 +85,61 @@ constexpr int KW = 32 + KD;      // samples in that window
 // outputs instead of once per output.
 // The row is an unpadded LDS array: every window read is an LDS load with an
 // immediate offset, so the loop is pure loads and FMAs.
+// The gather over the packed integer taps (load_taps2): two window samples
+// per tap word.
 template <int BATCH = 6>
-__device__ __forceinline__ void sinc4u(const float2 *row, int start, const float *tapl,
+__device__ __forceinline__ void sinc4p(const float2 *row, int start, const unsigned *tp,
                                        const int (&ph)[4], const int (&d)[4], float2 (&out)[4]) {
-    const float *col[4];
+    static_assert(KW % BATCH == 0 && BATCH % 2 == 0, "window splits into batches of pairs");
+    const unsigned *col[4];
     float ar[4], ai[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-        col[u] = tapl + (4 - d[u]) * 256 + scr(ph[u]);
+        const int base = 4 - d[u];                     // first tap row of output u
+        col[u] = tp + (base & 1) * TPH + (base >> 1) * 256 + scr(ph[u]);
         ar[u] = 0.f;
         ai[u] = 0.f;
     }
     const float2 *ws = row + start;
-    static_assert(KW % BATCH == 0, "window splits into batches");
-    // a rolled loop over batches: fully unrolled, the compiler front-loads all
-    // 180 window and tap loads and runs out of registers
 #pragma unroll 1
     for (int kb = 0; kb < KW; kb += BATCH) {
 #pragma unroll
-        for (int k = 0; k < BATCH; ++k) {
-            const float2 w = ws[k];
+        for (int k = 0; k < BATCH; k += 2) {
+            const float2 w0 = ws[k], w1 = ws[k + 1];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const float c = col[u][k * 256];
-                ar[u] = fmaf(c, w.x, ar[u]);
-                ai[u] = fmaf(c, w.y, ai[u]);
+                const unsigned v = col[u][(k / 2) * 256];
+                const float c0 = tap_lo(v), c1 = tap_hi(v);
+                ar[u] = fmaf(c0, w0.x, ar[u]);
+                ai[u] = fmaf(c0, w0.y, ai[u]);
+                ar[u] = fmaf(c1, w1.x, ar[u]);
+                ai[u] = fmaf(c1, w1.y, ai[u]);
             }
         }
         ws += BATCH;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) col[u] += BATCH * 256;
+        for (int u = 0; u < 4; ++u) col[u] += (BATCH / 2) * 256;
     }
+    constexpr float Q = 1.0f / 32768.0f;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) out[u] = make_float2(ar[u], ai[u]);
+    for (int u = 0; u < 4; ++u) out[u] = make_float2(ar[u] * Q, ai[u] * Q);
 }
 
-// SINC32 value of one interior output (bracket J, phase ph) read through fa.
+// SINC32 value of one interior output with the packed taps.
 template <typename FA>
-__device__ __forceinline__ float2 sinc1(FA &&fa, int J, int ph, const float *tapl) {
-    const float *col = tapl + 4 * 256 + scr(ph);
+__device__ __forceinline__ float2 sinc1p(FA &&fa, int J, int ph, const unsigned *tp) {
+    const unsigned *col = tp + 2 * 256 + scr(ph);      // rows 4 + 2i, 5 + 2i = taps 2i, 2i + 1
     float ar = 0.f, ai = 0.f;
-#pragma unroll 8
-    for (int t = 0; t < 32; ++t) {
-        const float2 s = fa(J - 15 + t);
-        ar = fmaf(col[t * 256], s.x, ar);
-        ai = fmaf(col[t * 256], s.y, ai);
+#pragma unroll 4
+    for (int i = 0; i < 16; ++i) {
+        const unsigned v = col[i * 256];
+        const float2 s0 = fa(J - 15 + 2 * i), s1 = fa(J - 14 + 2 * i);
+        ar = fmaf(tap_lo(v), s0.x, ar);
+        ai = fmaf(tap_lo(v), s0.y, ai);
+        ar = fmaf(tap_hi(v), s1.x, ar);
+        ai = fmaf(tap_hi(v), s1.y, ai);
     }
-    return make_float2(ar, ai);
+    return make_float2(ar * (1.0f / 32768.0f), ai * (1.0f / 32768.0f));
 }
 
 // Sum a per-thread count over the wave and add it once (fallback statistics).
@@ -173,10 +200,10 @@ __global__ __launch_bounds__(RT, 2) void k_range_f32(
     static_assert(sizeof(float2) * RQ * (RP + 1) <= sizeof(float2) * RP * RSP, "tile fits the strips");
     static_assert(RQ == 4 * 2 * (RT / RP), "two groups of four bins per thread");
     __shared__ float2 strip[RP * RSP];                 // source strips, then the output tile
-    __shared__ float tapl[SINC ? TROWS * 256 : 1];
+    __shared__ unsigned tp[SINC ? TPWORDS : 1];
     __shared__ int lo_s[RP], hi_s[RP], st_s[RP], dir_s[RP];
     auto tile = reinterpret_cast<float2 (*)[RP + 1]>(&strip[0]);
-    if (SINC) load_taps(tapl, sinc);
+    if (SINC) load_taps2(tp, sinc);
     const int tiles_p = (m + RP - 1) / RP, tiles_q = (n + RQ - 1) / RQ;
     const int my_groups = blockIdx.x < tiles_p ? (tiles_p - 1 - blockIdx.x) / gridDim.x + 1 : 0;
     unsigned slow = 0;
@@ -306,7 +333,7 @@ __global__ __launch_bounds__(RT, 2) void k_range_f32(
             fast = fast && J0 - 15 >= lo && J0 - 15 + KW <= hi;
             if (fast) {
                 float2 w4[4];
-                sinc4u(sp, J0 - 15 - lo, tapl, ph, d, w4);
+                sinc4p(sp, J0 - 15 - lo, tp, ph, d, w4);
 #pragma unroll
                 for (int u = 0; u < 4; ++u) v[g * 4 + u] = w4[u];
             } else {
@@ -321,7 +348,7 @@ __global__ __launch_bounds__(RT, 2) void k_range_f32(
                             if (code[k] & 255) r = fa(n - 1);
                         } else if (SINC && c - 1 >= 15 && c - 1 <= n - 17) {
                             ++slow;
-                            r = sinc1(fa, c - 1, code[k] & 255, tapl);
+                            r = sinc1p(fa, c - 1, code[k] & 255, tp);
                         } else {
                             const float2 a0 = fa(c - 1), a1 = fa(c);
                             const float fr = frq[k];
@@ -362,6 +389,9 @@ __device__ __forceinline__ int xcd_map(int b, int count) {
 // workgroup holds the LDS).
 template <int L>
 constexpr int az_emax() { return 8; }
+// The line FFT runs radix-16 passes (half the threads active) -- fewer LDS round
+// trips than radix 8 on all 1024.
+constexpr int AZ_FFT_EMAX = 16;
 template <int L>
 constexpr int az_nb() { return pfa32::Cfg<L, az_emax<L>()>::NB; }
 
@@ -414,13 +444,13 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
     constexpr int PR = (M2 + NB - 1) / NB;             // row samples per thread
     constexpr bool RES = MODE != 2;                    // resampling modes
     __shared__ float2 reg[Cf::PADN + 64];              // positions | row | FFT line
-    __shared__ float tapl[(SINC && RES) ? TROWS * 256 : 1];
+    __shared__ unsigned tp[(SINC && RES) ? TPWORDS : 1];
     __shared__ float2 twl[M2 / 64 + 64];
     __shared__ pfam::u16x8 optab[optab_len<L, FK>()];  // DFT-matrix operands (matrix-core FFT)
     __shared__ float wmax[NB / 64];
     double *xs = reinterpret_cast<double *>(reg);
     const int t = threadIdx.x;
-    if (SINC && RES) load_taps(tapl, sinc);
+    if (SINC && RES) load_taps2(tp, sinc);
     const pfa32::Tw tw = pfa32::load_tw<M2>(twl, twg);
     if constexpr (FK != FK_F32) pfam::build_ops_stage<L, BF, INV, 0>(optab);
     const int per_x = (n + 7) / 8;
@@ -526,7 +556,7 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
                 }
                 if (fast) {
                     float2 w4[4];
-                    sinc4u(reg, J0 - 15, tapl, ph, d, w4);
+                    sinc4p(reg, J0 - 15, tp, ph, d, w4);
 #pragma unroll
                     for (int u = 0; u < 4; ++u) v[g * 4 + u] = w4[u];
                 } else {
@@ -540,7 +570,7 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
                             if (c == m) {
                                 if (ph[u]) r = fa(m - 1);
                             } else if (SINC && c - 1 >= 15 && c - 1 <= m - 17) {
-                                r = sinc1(fa, c - 1, ph[u], tapl);
+                                r = sinc1p(fa, c - 1, ph[u], tp);
                             } else {
                                 const float2 f0 = fa(c - 1), f1 = fa(c);
                                 const float fr = frq[k];
@@ -634,7 +664,7 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
             __syncthreads();
             float2 *dst = out + ablk(j, 0, M2);
             const float scale = INV ? 1.0f / (float)M2 : 1.0f;
-            pfa32::fft<L, INV, EM>(reg, tw, [&](int idx, float2 w) {
+            pfa32::fft<L, INV, AZ_FFT_EMAX>(reg, tw, [&](int idx, float2 w) {
                 if (INV) w = make_float2(w.x * scale, w.y * scale);
                 st_stream(dst + idx, w);
             });
