@@ -87,10 +87,15 @@ constexpr int KW = 32 + KD;      // samples in that window
 // immediate offset, so the loop is pure loads and FMAs.
 // The gather over the packed integer taps (load_taps2): two window samples
 // per tap word.
-template <int BATCH = 6>
+// PAD16: the row is stored with one spare slot per 16 samples (pad16), so the
+// windows of lanes four samples apart start in different LDS banks; a batch of
+// BATCH <= 16 samples crosses at most one spare slot.
+__device__ __forceinline__ int pad16(int s) { return s + (s >> 4); }
+
+template <int BATCH = 6, bool PAD16 = false>
 __device__ __forceinline__ void sinc4p(const float2 *row, int start, const unsigned *tp,
                                        const int (&ph)[4], const int (&d)[4], float2 (&out)[4]) {
-    static_assert(KW % BATCH == 0 && BATCH % 2 == 0, "window splits into batches of pairs");
+    static_assert(KW % BATCH == 0 && BATCH % 2 == 0 && BATCH <= 16, "window splits into batches of pairs");
     const unsigned *col[4];
     float ar[4], ai[4];
 #pragma unroll
@@ -103,9 +108,17 @@ __device__ __forceinline__ void sinc4p(const float2 *row, int start, const unsig
     const float2 *ws = row + start;
 #pragma unroll 1
     for (int kb = 0; kb < KW; kb += BATCH) {
+        const float2 *wa = ws, *wb = ws;               // before / after the spare slot
+        int kbrk = BATCH;
+        if constexpr (PAD16) {
+            const int sb = start + kb;
+            wa = row + pad16(sb);
+            wb = wa + 1;
+            kbrk = 16 - (sb & 15);
+        }
 #pragma unroll
         for (int k = 0; k < BATCH; k += 2) {
-            const float2 w0 = ws[k], w1 = ws[k + 1];
+            const float2 w0 = (k < kbrk ? wa : wb)[k], w1 = (k + 1 < kbrk ? wa : wb)[k + 1];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const unsigned v = col[u][(k / 2) * 256];
@@ -155,7 +168,8 @@ constexpr int RQ = 256;                  // range bins per tile
 constexpr int RT = 512;                  // threads: 16 pulses x 32 groups of 4 bins, twice
 constexpr int RS = 304;                  // staged samples per pulse (RQ + window + slack)
 constexpr int RSP = RS + 1;              // odd pitch: a wave's 16 pulses hit different banks
-constexpr int RLD = (RP * RS + RT - 1) / RT;   // strip samples loaded per thread
+constexpr int RLD = (RS + 31) / 32;      // strip samples loaded per thread (32 threads a pulse)
+static_assert(RT == 32 * RP, "32 strip-loading threads per pulse");
 
 // Exact count c = #{u : xa(u) <= Q} over an ascending view of S positions,
 // walking from a guess (a few steps), bisecting when the guess is far off;
@@ -269,15 +283,18 @@ __global__ __launch_bounds__(RT, 2) void k_range_f32(
             dir_s[t] = asc ? 1 : -1;
         }
         __syncthreads();
-        // strip loads in flight while the brackets are found
+        // strip loads in flight while the brackets are found: 32 threads per
+        // pulse, thread samples u = t % 32 + 32 k (consecutive lanes, consecutive samples)
         float2 pre[RLD];
+        {
+            const int sp_ = t / 32, su = t % 32;
+            const int slo = lo_s[sp_], shi = hi_s[sp_], sst = st_s[sp_], sdir = dir_s[sp_];
+            const float2 *srow = sig + (size_t)(p0 + sp_ < m ? order[p0 + sp_] : 0) * n + sst;
 #pragma unroll
-        for (int k = 0; k < RLD; ++k) {
-            const int r = t + k * RT;
-            const int pp = r / RS, u = r - (r / RS) * RS;
-            pre[k] = make_float2(0.f, 0.f);
-            if (pp < RP && lo_s[pp] + u < hi_s[pp])
-                pre[k] = ld_stream(sig + (size_t)order[p0 + pp] * n + st_s[pp] + dir_s[pp] * u);
+            for (int k = 0; k < RLD; ++k) {
+                const int u = su + 32 * k;
+                pre[k] = (u < RS && slo + u < shi) ? ld_stream(srow + sdir * u) : make_float2(0.f, 0.f);
+            }
         }
         // brackets of this thread's bins: (c << 8) | phase and the fraction
         int code[8];
@@ -307,9 +324,8 @@ __global__ __launch_bounds__(RT, 2) void k_range_f32(
         }
 #pragma unroll
         for (int k = 0; k < RLD; ++k) {
-            const int r = t + k * RT;
-            const int pp = r / RS, u = r - (r / RS) * RS;
-            if (pp < RP) strip[pp * RSP + u] = pre[k];
+            const int u = t % 32 + 32 * k;
+            if (u < RS) strip[(t / 32) * RSP + u] = pre[k];
         }
         __syncthreads();
         // gather
@@ -443,7 +459,7 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
     constexpr int G = (M2 + 4 * NB - 1) / (4 * NB);   // 4-output groups per thread
     constexpr int PR = (M2 + NB - 1) / NB;             // row samples per thread
     constexpr bool RES = MODE != 2;                    // resampling modes
-    __shared__ float2 reg[Cf::PADN + 64];              // positions | row | FFT line
+    __shared__ float2 reg[(M2 + M2 / 16 + 64 > Cf::PADN + 64) ? M2 + M2 / 16 + 64 : Cf::PADN + 64];  // positions | row | FFT line
     __shared__ unsigned tp[(SINC && RES) ? TPWORDS : 1];
     __shared__ float2 twl[M2 / 64 + 64];
     __shared__ pfam::u16x8 optab[optab_len<L, FK>()];  // DFT-matrix operands (matrix-core FFT)
@@ -532,16 +548,16 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
                 }
             }
             __syncthreads();                          // positions dead: the row takes over
-            // (3) the row (unpadded), zero past m so shifted windows read zeros
+            // (3) the row (pad16 layout), zero past m so shifted windows read zeros
 #pragma unroll
             for (int k = 0; k < PR; ++k) {
                 const int p = t + k * NB;
-                if (p < M2) reg[p] = pre[k];
+                if (p < M2) reg[pad16(p)] = pre[k];
             }
-            if (t < 48) reg[m + t] = make_float2(0.f, 0.f);
+            if (t < 48) reg[pad16(m + t)] = make_float2(0.f, 0.f);
             __syncthreads();
             // (4) gather
-            auto fa = [&](int u) { return reg[u]; };
+            auto fa = [&](int u) { return reg[pad16(u)]; };
 #pragma unroll
             for (int g = 0; g < G; ++g) {
                 int d[4], ph[4];
@@ -556,7 +572,7 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
                 }
                 if (fast) {
                     float2 w4[4];
-                    sinc4p(reg, J0 - 15, tp, ph, d, w4);
+                    sinc4p<6, true>(reg, J0 - 15, tp, ph, d, w4);
 #pragma unroll
                     for (int u = 0; u < 4; ++u) v[g * 4 + u] = w4[u];
                 } else {
