@@ -16,8 +16,8 @@
 //
 // Data between stages lives in LDS as 32-bit words of (re, im) 16-bit pairs,
 // stored vector-major for the stage that reads it: element e = k + r N/R of a
-// radix-R stage sits at word k R + r, padded by 4 words every 32 so the
-// 16-byte operand reads of 8 lanes (64 or 128 bytes apart) use distinct banks.
+// radix-R stage sits at word k R + r, padded (pp) so the 16-byte operand reads
+// and the scattered result writes of a wave use distinct banks.
 // The twiddle of the next stage is applied in FP32 to the accumulators before
 // they are rounded and stored (one 32-bit LDS write per complex output).
 //
@@ -110,10 +110,14 @@ constexpr int ls_of() {                      // product of the radices before st
     return p;
 }
 
-// Padded word of position p in a line; line length in words incl. padding.
-__device__ __forceinline__ int pp(int p) { return p + ((p >> 5) << 2); }
+// Padded word of position p in a line; line length in words incl. padding
+// (a multiple of 4 words, so every line starts 16-byte aligned).  4 words every
+// 32 spread the 16-byte operand reads of a wave over the banks; 4 more every
+// 512 spread the scattered 32-bit result writes, whose lanes land 512 words
+// apart (same bank) under the first rule alone.
+__device__ __forceinline__ int pp(int p) { return p + ((p >> 5) << 2) + ((p >> 9) << 2); }
 template <int L>
-constexpr int line_words() { return (1 << L) + ((1 << L) >> 3); }
+constexpr int line_words() { return ((1 << L) + ((1 << L) >> 3) + (((1 << L) >> 9) << 2) + 3) & ~3; }
 
 // Position of element e in the plane layout read by stage S.
 template <int L, int S>
@@ -169,33 +173,10 @@ constexpr int tile0() {
     return t;
 }
 
-// DFT-matrix operands computed in registers for each stage call.
-template <int L, bool BF, bool INV>
-struct OpsReg {
-    template <int S>
-    __device__ __forceinline__ void get(int mt, int ks, int lane, typename H16<BF>::V &hi,
-                                        typename H16<BF>::V &lo) const {
-        dft_operand<MPlan<L>::R[S], BF, INV>(mt, ks, lane, hi, lo);
-    }
-};
-
-// DFT-matrix operands from a table built once per workgroup (build_ops):
-// tab[t * 64 + lane] = hi of tile t, tab[(ntiles + t) * 64 + lane] = lo.
-template <int L, bool BF, bool INV>
-struct OpsLds {
-    const u16x8 *tab;
-    template <int S>
-    __device__ __forceinline__ void get(int mt, int ks, int lane, typename H16<BF>::V &hi,
-                                        typename H16<BF>::V &lo) const {
-        constexpr int KS = 2 * MPlan<L>::R[S] / 32;
-        const int t = tile0<L, S>() + mt * KS + ks;
-        hi = __builtin_bit_cast(typename H16<BF>::V, tab[t * 64 + lane]);
-        lo = __builtin_bit_cast(typename H16<BF>::V, tab[(ntiles<L>() + t) * 64 + lane]);
-    }
-};
-
-// The same table in global memory (built once per size and precision by
-// k_build_ops, read through the caches): 16 bytes per lane and tile.
+// DFT-matrix operands from a table in global memory, built once per size,
+// precision and direction (build_ops_stage), read through the caches: 16 bytes
+// per lane and tile, tab[t * 64 + lane] = hi of tile t, tab[(ntiles + t) * 64 +
+// lane] = lo.
 template <int L, bool BF, bool INV>
 struct OpsGlobal {
     const u16x8 *__restrict__ tab;
@@ -222,11 +203,6 @@ __device__ __forceinline__ void build_ops_stage(u16x8 *tab) {
         }
         build_ops_stage<L, BF, INV, S + 1>(tab);
     }
-}
-template <int L, bool BF, bool INV>
-__device__ __forceinline__ OpsLds<L, BF, INV> build_ops(u16x8 *tab) {
-    build_ops_stage<L, BF, INV, 0>(tab);
-    return OpsLds<L, BF, INV>{tab};
 }
 
 // One stage over NCOL lines held in `planes` (line_words<L>() words per line).  Stage S reads its layout, applies the

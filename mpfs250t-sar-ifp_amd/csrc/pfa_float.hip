@@ -69,20 +69,27 @@ int launch_float_azimuth(pfa_plan *p, int mode, const void *g1t, int ldp, const 
     pfa_timer_begin(ctx, PFA_TIMER_AZIMUTH);
     const TanBins tb{p->tan_bin, p->tb_lo, p->tb_inv_w, p->tb_bins};
 #define ARGS (const float2 *)g1t, ldp, p->m, p->n, grid_of(p->KR, p->kr_lin, p->n), p->tan_s64, tb, grid_of(p->KC, p->kc_lin, p->m), p->hm, p->hn, p->sinc, \
-             (const float2 *)g2in, (float2 *)out, tw, (unsigned *)(ctx->d_counters + 32), nullptr, nullptr, (const float2 *)rot
+             (const float2 *)g2in, (float2 *)out, tw, (unsigned *)(ctx->d_counters + 32), nullptr, nullptr, (const float2 *)rot, nullptr
 #define K4(LL, MODE)                                                                                 \
     if (sinc && inv) hipLaunchKernelGGL((k_azimuth_f32<LL, MODE, true, true>), dim3(grid_az), dim3(az_nb<LL>()), 0, ctx->stream, ARGS); \
     else if (sinc) hipLaunchKernelGGL((k_azimuth_f32<LL, MODE, true, false>), dim3(grid_az), dim3(az_nb<LL>()), 0, ctx->stream, ARGS); \
     else if (inv) hipLaunchKernelGGL((k_azimuth_f32<LL, MODE, false, true>), dim3(grid_az), dim3(az_nb<LL>()), 0, ctx->stream, ARGS); \
     else hipLaunchKernelGGL((k_azimuth_f32<LL, MODE, false, false>), dim3(grid_az), dim3(az_nb<LL>()), 0, ctx->stream, ARGS);
-#define LAUNCH(LL)                       \
-    case LL:                             \
-        if (mode == 0) { K4(LL, 0) }     \
-        else if (mode == 1) { K4(LL, 1) } \
-        else { K4(LL, 2) }               \
+#define K4R(LL)                                                                                      \
+    if (sinc && inv) hipLaunchKernelGGL((k_azimuth_f32<LL, 1, true, true, FK_F32, true>), dim3(grid_az), dim3(az_nb<LL>()), 0, ctx->stream, ARGS); \
+    else if (sinc) hipLaunchKernelGGL((k_azimuth_f32<LL, 1, true, false, FK_F32, true>), dim3(grid_az), dim3(az_nb<LL>()), 0, ctx->stream, ARGS); \
+    else if (inv) hipLaunchKernelGGL((k_azimuth_f32<LL, 1, false, true, FK_F32, true>), dim3(grid_az), dim3(az_nb<LL>()), 0, ctx->stream, ARGS); \
+    else hipLaunchKernelGGL((k_azimuth_f32<LL, 1, false, false, FK_F32, true>), dim3(grid_az), dim3(az_nb<LL>()), 0, ctx->stream, ARGS);
+#define LAUNCH(LL)                          \
+    case LL:                                \
+        if (mode == 0) { K4(LL, 0) }        \
+        else if (mode == 1 && rot) { K4R(LL) } \
+        else if (mode == 1) { K4(LL, 1) }   \
+        else { K4(LL, 2) }                  \
         break;
     PFA_F32_SWITCH(L, LAUNCH)
 #undef LAUNCH
+#undef K4R
 #undef K4
 #undef ARGS
     pfa_timer_end(ctx, PFA_TIMER_AZIMUTH);
@@ -102,10 +109,10 @@ int launch_float_fft_b(pfa_plan *p, const void *A, void *out, int out_kind) {
 #define ARGS (const float2 *)A, p->n, p->m2, out, tw
 #define LAUNCH(LL)                                                                                   \
     case LL:                                                                                         \
-        if (inv && out_kind) hipLaunchKernelGGL((k_fft_b2_f32<LL, true, 1>), dim3(p->m2 / 2), dim3(nb_of<LL>()), 0, ctx->stream, ARGS); \
-        else if (inv) hipLaunchKernelGGL((k_fft_b2_f32<LL, true, 0>), dim3(p->m2 / 2), dim3(nb_of<LL>()), 0, ctx->stream, ARGS); \
-        else if (out_kind) hipLaunchKernelGGL((k_fft_b2_f32<LL, false, 1>), dim3(p->m2 / 2), dim3(nb_of<LL>()), 0, ctx->stream, ARGS); \
-        else hipLaunchKernelGGL((k_fft_b2_f32<LL, false, 0>), dim3(p->m2 / 2), dim3(nb_of<LL>()), 0, ctx->stream, ARGS); \
+        if (inv && out_kind) hipLaunchKernelGGL((k_fft_b2_f32<LL, true, 1>), dim3(p->m2 / 2), dim3(fb2_nb<LL>()), 0, ctx->stream, ARGS); \
+        else if (inv) hipLaunchKernelGGL((k_fft_b2_f32<LL, true, 0>), dim3(p->m2 / 2), dim3(fb2_nb<LL>()), 0, ctx->stream, ARGS); \
+        else if (out_kind) hipLaunchKernelGGL((k_fft_b2_f32<LL, false, 1>), dim3(p->m2 / 2), dim3(fb2_nb<LL>()), 0, ctx->stream, ARGS); \
+        else hipLaunchKernelGGL((k_fft_b2_f32<LL, false, 0>), dim3(p->m2 / 2), dim3(fb2_nb<LL>()), 0, ctx->stream, ARGS); \
         break;
     PFA_F32_SWITCH(L, LAUNCH)
 #undef LAUNCH

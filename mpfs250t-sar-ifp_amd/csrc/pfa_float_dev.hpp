@@ -216,6 +216,8 @@ __global__ __launch_bounds__(RT, 2) void k_range_f32(
     __shared__ float2 strip[RP * RSP];                 // source strips, then the output tile
     __shared__ unsigned tp[SINC ? TPWORDS : 1];
     __shared__ int lo_s[RP], hi_s[RP], st_s[RP], dir_s[RP];
+    __shared__ int ps_i[RP];                          // per pulse of the current group: sorted index
+    __shared__ double ps_a[RP], ps_f0[RP], ps_df[RP], ps_pr[RP];  // 2 pr / c, f0, df, pr
     auto tile = reinterpret_cast<float2 (*)[RP + 1]>(&strip[0]);
     if (SINC) load_taps2(tp, sinc);
     const int tiles_p = (m + RP - 1) / RP, tiles_q = (n + RQ - 1) / RQ;
@@ -228,27 +230,48 @@ __global__ __launch_bounds__(RT, 2) void k_range_f32(
         asm volatile("" : "+v"(t));
         const int pl = t % RP, gi = t / RP;           // pulse in tile, bin group
         const int p = p0 + pl;
-        // per-pulse position model kr(j) = (2 pr / c) f(j), f(j) = f0 + j df
-        int i = 0;
-        double ai = 1.0, f0 = 0.0, df = 1.0;
-        if (p < m) {
-            i = order[p];
-            ai = a64[i];
-            if (FULLFREQ) {
-                f0 = freq[(size_t)i * n];
-                df = __ddiv_rn(__dsub_rn(freq[(size_t)i * n + n - 1], f0), (double)(n - 1));
-            } else {
-                f0 = f0v[i];
-                df = dfv[i];
+        if (q0 == 0) {                                // first tile of a pulse group: its pulse values to LDS
+            __syncthreads();                          // the previous group's tiles are done with them
+            if (t < RP) {
+                int ii = 0;
+                double aa = 1.0, ff = 0.0, dd = 1.0, pv = 1.0;
+                if (p0 + t < m) {
+                    ii = order[p0 + t];
+                    aa = a64[ii];
+                    pv = pr64[ii];
+                    if (FULLFREQ) {
+                        ff = freq[(size_t)ii * n];
+                        dd = __ddiv_rn(__dsub_rn(freq[(size_t)ii * n + n - 1], ff), (double)(n - 1));
+                    } else {
+                        ff = f0v[ii];
+                        dd = dfv[ii];
+                    }
+                }
+                ps_i[t] = ii;
+                ps_a[t] = aa;
+                ps_f0[t] = ff;
+                ps_df[t] = dd;
+                ps_pr[t] = pv;
             }
+            __syncthreads();
         }
+        // grid values of this thread's bins first: loads of a KR table complete
+        // before the strip loads start, so the bracket search overlaps them
+        double Qv[8];
+#pragma unroll
+        for (int g = 0; g < 2; ++g)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) Qv[g * 4 + u] = KR(min(q0 + 4 * (gi + g * (RT / RP)) + u, n - 1));
+        // per-pulse position model kr(j) = (2 pr / c) f(j), f(j) = f0 + j df
+        const int i = ps_i[pl];
+        const double ai = ps_a[pl], f0 = ps_f0[pl], df = ps_df[pl];
         const double *frow = freq + (size_t)i * n;
         auto fq = [&](int j) { return FULLFREQ ? frow[j] : __dadd_rn(f0, __dmul_rn((double)j, df)); };
         // kr as (2 pr / c) f: one product per position, within a few ulp of numpy's
         // (2 f / c) pr; kr_np is numpy's rounding exactly, used to decide near-ties
         // (the KR grid's end points are such ties by construction)
         auto kr = [&](int j) { return __dmul_rn(ai, fq(j)); };
-        const double prv = p < m ? pr64[i] : 1.0;
+        const double prv = ps_pr[pl];
         auto kr_np = [&](int j) { return __dmul_rn(__ddiv_rn(2.0 * fq(j), C_LIGHT), prv); };
         const bool asc = p < m ? !(kr_np(0) > kr_np(n - 1)) : true;
         auto xa = [&](int u) { return kr(asc ? u : n - 1 - u); };
@@ -284,17 +307,18 @@ __global__ __launch_bounds__(RT, 2) void k_range_f32(
         }
         __syncthreads();
         // strip loads in flight while the brackets are found: 32 threads per
-        // pulse, thread samples u = t % 32 + 32 k (consecutive lanes, consecutive samples)
+        // pulse, thread samples u = t % 32 + 32 k (consecutive lanes, consecutive
+        // samples), issued unconditionally from clamped slots; slots past the
+        // strip are zeroed when stored (overwriting a register whose load is
+        // in flight would wait for it)
         float2 pre[RLD];
         {
             const int sp_ = t / 32, su = t % 32;
             const int slo = lo_s[sp_], shi = hi_s[sp_], sst = st_s[sp_], sdir = dir_s[sp_];
-            const float2 *srow = sig + (size_t)(p0 + sp_ < m ? order[p0 + sp_] : 0) * n + sst;
+            const float2 *srow = sig + (size_t)ps_i[sp_] * n + sst;
+            const int ulast = max(shi - slo - 1, 0);
 #pragma unroll
-            for (int k = 0; k < RLD; ++k) {
-                const int u = su + 32 * k;
-                pre[k] = (u < RS && slo + u < shi) ? ld_stream(srow + sdir * u) : make_float2(0.f, 0.f);
-            }
+            for (int k = 0; k < RLD; ++k) pre[k] = ld_stream(srow + sdir * min(su + 32 * k, ulast));
         }
         // brackets of this thread's bins: (c << 8) | phase and the fraction
         int code[8];
@@ -308,7 +332,7 @@ __global__ __launch_bounds__(RT, 2) void k_range_f32(
                 code[k] = 0;
                 frq[k] = 0.f;
                 if (p >= m || q >= qe) continue;
-                const double Q = KR(q);
+                const double Q = Qv[k];
                 double xl, xh;
                 const int c = bracket(Q, xl, xh);
                 int ph = 0;
@@ -322,10 +346,13 @@ __global__ __launch_bounds__(RT, 2) void k_range_f32(
                 code[k] = (c << 8) | ph;
             }
         }
+        {
+            const int sp_ = t / 32, slo = lo_s[sp_], shi = hi_s[sp_];
 #pragma unroll
-        for (int k = 0; k < RLD; ++k) {
-            const int u = t % 32 + 32 * k;
-            if (u < RS) strip[(t / 32) * RSP + u] = pre[k];
+            for (int k = 0; k < RLD; ++k) {
+                const int u = t % 32 + 32 * k;
+                if (u < RS) strip[sp_ * RSP + u] = slo + u < shi ? pre[k] : make_float2(0.f, 0.f);
+            }
         }
         __syncthreads();
         // gather
@@ -416,10 +443,11 @@ struct TanBins {
     const int32_t *idx;
     double lo, inv_w;
     int bins;
-    __device__ __forceinline__ int guess(double t) const {
+    __device__ __forceinline__ int bin(double t) const {
         const double b = floor(__dmul_rn(__dsub_rn(t, lo), inv_w));
-        return idx[(b == b) ? (int)fmin(fmax(b, 0.0), (double)bins) : 0];
+        return (b == b) ? (int)fmin(fmax(b, 0.0), (double)bins) : 0;
     }
+    __device__ __forceinline__ int guess(double t) const { return idx[bin(t)]; }
 };
 
 // Persistent over range bins: iteration it of workgroup b (XCD x = b % 8, rank
@@ -435,20 +463,15 @@ struct TanBins {
 // 2 FP16, 3 BF16 with hi/lo DFT matrices -- matrix-core FFT (pfa_fft_mfma.hpp),
 // A stored as packed 16-bit pairs with a per-row exponent.
 constexpr int FK_F32 = 0, FK_BF16 = 1, FK_FP16 = 2, FK_BF16S = 3;
-template <int L, int FK>
-constexpr int optab_len() {
-    if constexpr (FK == FK_F32) return 1;
-    else return 2 * pfam::ntiles<L>() * 64;
-}
-
-template <int L, int MODE, bool SINC, bool INV, int FK = FK_F32>
+template <int L, int MODE, bool SINC, bool INV, int FK = FK_F32, bool ROT = false>
 __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
     const float2 *__restrict__ g1t, int ldp, int m, int n, const Grid1 KR,
     const double *__restrict__ tan_s, const TanBins tb, const Grid1 KC,
     const double *__restrict__ hm, const double *__restrict__ hn,
     const int16_t *__restrict__ sinc, const float2 *__restrict__ g2in,
     float2 *__restrict__ out, const float2 *__restrict__ twg, unsigned *__restrict__ nslow,
-    unsigned *__restrict__ A16, int *__restrict__ expA, const float2 *__restrict__ rot) {
+    unsigned *__restrict__ A16, int *__restrict__ expA, const float2 *__restrict__ rot,
+    const pfam::u16x8 *__restrict__ ops) {
     static_assert(FK == FK_F32 || MODE == 1, "matrix-core FFT only in the fused mode");
     constexpr bool BF = FK == FK_BF16 || FK == FK_BF16S;
     constexpr bool SPLIT = FK == FK_BF16S;
@@ -462,29 +485,31 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
     __shared__ float2 reg[(M2 + M2 / 16 + 64 > Cf::PADN + 64) ? M2 + M2 / 16 + 64 : Cf::PADN + 64];  // positions | row | FFT line
     __shared__ unsigned tp[(SINC && RES) ? TPWORDS : 1];
     __shared__ float2 twl[M2 / 64 + 64];
-    __shared__ pfam::u16x8 optab[optab_len<L, FK>()];  // DFT-matrix operands (matrix-core FFT)
     __shared__ float wmax[NB / 64];
+    __shared__ float wl[RES ? G * 4 * NB : 1];         // this line's window weights, thread-private slots
     double *xs = reinterpret_cast<double *>(reg);
     const int t = threadIdx.x;
     if (SINC && RES) load_taps2(tp, sinc);
     const pfa32::Tw tw = pfa32::load_tw<M2>(twl, twg);
-    if constexpr (FK != FK_F32) pfam::build_ops_stage<L, BF, INV, 0>(optab);
     const int per_x = (n + 7) / 8;
     const int ranks = (gridDim.x + 7) / 8;
     const int x = blockIdx.x % 8, rank = blockIdx.x / 8;
     auto line_of = [&](int base) { return x * per_x + base + rank; };
     auto active_at = [&](int base) { return base + rank < per_x && line_of(base) < n; };
     unsigned slow = 0;
-    float2 pre[PR];                                    // next row, in flight
-    // rot (optional): per sorted pulse e^{j phi} of a phase correction (autofocus)
+    // next row, in flight: loaded unconditionally from clamped positions and
+    // masked when stored (overwriting a register whose load is in flight would
+    // wait for it); rot (ROT): per sorted pulse e^{j phi} of a phase correction
+    // (autofocus), applied after the row is stored.  Loads on paths taken at
+    // run time only (a null-pointer test) would leave registers the compiler
+    // must wait for after every later join, the row prefetch included.
+    float2 pre[PR];
     auto prefetch = [&](int j) {
         const float2 *row = g1t + (size_t)j * ldp;
+        int tp_ = t;                                   // opaque: addresses are not held across lines
+        asm volatile("" : "+v"(tp_));
 #pragma unroll
-        for (int k = 0; k < PR; ++k) {
-            const int p = t + k * NB;
-            pre[k] = p < m ? ld_stream(row + p) : make_float2(0.f, 0.f);
-            if (rot && p < m) pre[k] = pfa32::cmul(pre[k], rot[p]);
-        }
+        for (int k = 0; k < PR; ++k) pre[k] = ld_stream(row + min(tp_ + k * NB, m - 1));
     };
     if (RES && active_at(0)) prefetch(line_of(0));
     for (int base = 0; base < per_x; base += ranks) {
@@ -505,12 +530,30 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
         } else {
             const double krj = KR(j);
             const double inv_krj = 1.0 / krj;
-            // (1) FP64 source positions of this line
+            const double hj = hn ? hn[j] : 1.0;
+            // (1) FP64 source positions of this line (all loads in flight before
+            // the first product)
+            double tv[PR];
+#pragma unroll
+            for (int k = 0; k < PR; ++k) tv[k] = tan_s[min(tl + k * NB, m - 1)];
             __syncthreads();                          // previous line fully consumed
-            for (int p = t; p < m; p += NB) xs[p] = __dmul_rn(krj, tan_s[p]);
+#pragma unroll
+            for (int k = 0; k < PR; ++k)
+                if (tl + k * NB < m) xs[tl + k * NB] = __dmul_rn(krj, tv[k]);
             __syncthreads();
             // (2) brackets: c = #{p : xs[p] <= Q} (np.interp / searchsorted-right),
-            // kept as (c << 8) | sinc phase, and the interval fraction
+            // kept as (c << 8) | sinc phase, and the interval fraction.  The
+            // tan-bin guesses and the window's taper values are loaded for all
+            // outputs first, so their global round trips overlap.
+            int gix[G * 4];
+#pragma unroll
+            for (int k = 0; k < G * 4; ++k) gix[k] = tb.bin(__dmul_rn(KC(min(((k / 4) * NB + tl) * 4 + k % 4, m - 1)), inv_krj));
+            double hmv[G * 4];
+#pragma unroll
+            for (int k = 0; k < G * 4; ++k) {
+                gix[k] = tb.idx[gix[k]];
+                hmv[k] = hm ? hm[min(((k / 4) * NB + tl) * 4 + k % 4, m - 1)] : 1.0;
+            }
             int code[G * 4];
             float frq[G * 4];
 #pragma unroll
@@ -521,9 +564,10 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
                     const int k = g * 4 + u;
                     code[k] = 0;
                     frq[k] = 0.f;
+                    wl[k * NB + t] = (float)__dmul_rn(hmv[k], hj);
                     if (q >= m) continue;
                     const double Q = KC(q);
-                    int c = tb.guess(__dmul_rn(Q, inv_krj));
+                    int c = gix[k];
                     double xl = c > 0 ? xs[c - 1] : -INFINITY;
                     double xh = c < m ? xs[c] : INFINITY;
                     if (xh <= Q) {                    // the guess is low by one for ~1/4 of outputs
@@ -552,7 +596,17 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
 #pragma unroll
             for (int k = 0; k < PR; ++k) {
                 const int p = t + k * NB;
-                if (p < M2) reg[pad16(p)] = pre[k];
+                if (p < M2) reg[pad16(p)] = p < m ? pre[k] : make_float2(0.f, 0.f);
+            }
+            if constexpr (ROT) {                      // this thread's own slots: no barrier needed
+                float2 rv[PR];
+#pragma unroll
+                for (int k = 0; k < PR; ++k) rv[k] = rot[min(tl + k * NB, m - 1)];
+#pragma unroll
+                for (int k = 0; k < PR; ++k) {
+                    const int p = tl + k * NB;
+                    if (p < m) reg[pad16(p)] = pfa32::cmul(reg[pad16(p)], rv[k]);
+                }
             }
             if (t < 48) reg[pad16(m + t)] = make_float2(0.f, 0.f);
             __syncthreads();
@@ -603,16 +657,11 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
                 if (nb2 < per_x && active_at(nb2)) prefetch(line_of(nb2));
             }
             // window
-            const double hj = hn ? hn[j] : 1.0;
+            if (hm) {
 #pragma unroll
-            for (int g = 0; g < G; ++g) {
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int q = (g * NB + tl) * 4 + u;
-                    if (q < m && hm) {
-                        const float wgt = (float)__dmul_rn(hm[q], hj);
-                        v[g * 4 + u] = make_float2(v[g * 4 + u].x * wgt, v[g * 4 + u].y * wgt);
-                    }
+                for (int k = 0; k < G * 4; ++k) {
+                    const float w = wl[k * NB + t];
+                    v[k] = make_float2(v[k].x * w, v[k].y * w);
                 }
             }
             if (MODE == 0) {
@@ -655,7 +704,7 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
                 __syncthreads();
                 constexpr int NW = pfam::nw_for<L, 1, NB / 64>();
                 unsigned *dst = A16 + (size_t)j * M2;
-                pfam::fft<L, BF, INV, SPLIT, 1, NW>(planes, tw, pfam::OpsLds<L, BF, INV>{optab},
+                pfam::fft<L, BF, INV, SPLIT, 1, NW>(planes, tw, pfam::OpsGlobal<L, BF, INV>{ops},
                     [&](int, int o, float2 y) {
                         __builtin_nontemporal_store(pfam::pack<BF>(y), dst + o);
                     });
@@ -697,11 +746,16 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
 // Thread t holds elements t + s NT (s < E) of both columns -- exactly the
 // inputs of its first Stockham pass -- transforms the first column through
 // the LDS line, then the second from its registers through the same line.
+// Radix-16 passes (16 elements a thread) keep a workgroup within 128 VGPRs,
+// so two share a CU and one's loads overlap the other's passes.
+constexpr int FB_EMAX = 16;
+template <int L>
+constexpr int fb2_nb() { return pfa32::Cfg<L, FB_EMAX>::NT < 64 ? 64 : pfa32::Cfg<L, FB_EMAX>::NT; }
 template <int L, bool INV, int OUT>
-__global__ __launch_bounds__(pfa32::Cfg<L>::NT < 64 ? 64 : pfa32::Cfg<L>::NT, 2) void k_fft_b2_f32(
+__global__ __launch_bounds__(fb2_nb<L>(), 4) void k_fft_b2_f32(
     const float2 *__restrict__ A, int n, int m2, void *__restrict__ outp,
     const float2 *__restrict__ twg) {
-    using Cf = pfa32::Cfg<L>;
+    using Cf = pfa32::Cfg<L, FB_EMAX>;
     constexpr int N2 = Cf::N, NT = Cf::NT, E = Cf::E;
     __shared__ float2 buf[Cf::PADN];
     __shared__ float2 twl[N2 / 64 + 64];
@@ -727,8 +781,8 @@ __global__ __launch_bounds__(pfa32::Cfg<L>::NT < 64 ? 64 : pfa32::Cfg<L>::NT, 2)
             else reinterpret_cast<float *>(outp)[(size_t)r * N2 + c] = __fsqrt_rn(v.x * v.x + v.y * v.y);
         };
     };
-    pfa32::fft_src<L, INV>(buf, tw, [&](int, int s2) { return c0[s2]; }, sink_for(r0));
-    pfa32::fft_src<L, INV>(buf, tw, [&](int, int s2) { return c1[s2]; }, sink_for(r0 + 1));
+    pfa32::fft_src<L, INV, FB_EMAX>(buf, tw, [&](int, int s2) { return c0[s2]; }, sink_for(r0));
+    pfa32::fft_src<L, INV, FB_EMAX>(buf, tw, [&](int, int s2) { return c1[s2]; }, sink_for(r0 + 1));
 }
 
 // Two-level twiddle tables of an n-point line: hi[k] = W^(64 k), k < n/64,

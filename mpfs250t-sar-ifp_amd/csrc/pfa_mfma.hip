@@ -20,7 +20,7 @@ namespace {
 template <int L>
 constexpr int fftb_mfma_nb() { return 64 * pfam::nw_for<L, 4, 8>(); }
 
-// DFT-matrix operand table of an L-point matrix-core FFT (pfam::build_ops layout).
+// DFT-matrix operand table of an L-point matrix-core FFT (pfam::OpsGlobal layout).
 template <int L, bool BF, bool INV>
 __global__ __launch_bounds__(256) void k_build_ops(pfam::u16x8 *tab) {
     pfam::build_ops_stage<L, BF, INV, 0>(tab);
@@ -52,19 +52,31 @@ __global__ __launch_bounds__(fftb_mfma_nb<L>()) void k_fft_b_mfma(
 #pragma unroll
     for (int i = 0; i < BW; ++i) S = max(S, emx[i]);
     S = (S == INT_MIN) ? 0 : S + 2;
-    // the corner-turn: 4 adjacent cross bins of every range row, rescaled to 2^-S
-    for (int jj = t; jj < N2; jj += NB) {
-        uint4 a = make_uint4(0u, 0u, 0u, 0u);
-        float sc = 0.f;
-        if (jj < n) {
-            a = *reinterpret_cast<const uint4 *>(A16 + (size_t)jj * m2 + r0);
-            sc = ldexpf(1.0f, expA[jj] - S);
-        }
-        const unsigned wv[4] = {a.x, a.y, a.z, a.w};
+    // the corner-turn: 4 adjacent cross bins of every range row, rescaled to
+    // 2^-S.  Lane t takes row k + r NV0 (r = t % R0): consecutive lanes write
+    // consecutive words of the first-stage layout (rows are 4 m2 bytes apart in
+    // A16, so the assignment does not change the global requests).
+    constexpr int R0 = pfam::MPlan<L>::R[0], NV0 = N2 / R0, NI = N2 / NB;
+    static_assert(NB % R0 == 0 && N2 % NB == 0, "row assignment covers the line");
+    auto row_of = [&](int i) { return t / R0 + i * (NB / R0) + (t % R0) * NV0; };
+    uint4 a[NI];                                       // all loads in flight first (clamped rows)
+    int ex[NI];
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+        const int jc = min(row_of(i), n - 1);
+        a[i] = *reinterpret_cast<const uint4 *>(A16 + (size_t)jc * m2 + r0);
+        ex[i] = expA[jc];
+    }
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+        const int jj = row_of(i);
+        const bool in = jj < n;
+        const float sc = ldexpf(1.0f, ex[i] - S);
+        const unsigned wv[4] = {a[i].x, a[i].y, a[i].z, a[i].w};
 #pragma unroll
         for (int c = 0; c < NCOL; ++c) {
             const float2 v = pfam::unpack<BF>(wv[c]);
-            pfam::put<L, BF>(planes, c, jj, make_float2(v.x * sc, v.y * sc));
+            pfam::put<L, BF>(planes, c, jj, in ? make_float2(v.x * sc, v.y * sc) : make_float2(0.f, 0.f));
         }
     }
     __syncthreads();
@@ -86,44 +98,6 @@ __global__ __launch_bounds__(fftb_mfma_nb<L>()) void k_fft_b_mfma(
         MACRO(8) MACRO(9) MACRO(10) MACRO(12) MACRO(13)                                           \
         default: return pfa_fail(ctx, PFA_ERR_DIMS, "matrix-core FFT sizes are 256, 512, 1024, 4096 and 8192"); \
     }
-
-int launch_float_azimuth_mfma(pfa_plan *p, const void *g1t, int ldp, unsigned *A16, int *expA,
-                              int fk) {
-    pfa_ctx *ctx = p->ctx;
-    int st = PFA_OK;
-    const int m2 = p->m2;
-    const float2 *tw = twiddles_f32(ctx, m2, &st);
-    if (!tw) return st;
-    const bool sinc = p->flags & PFA_FLOAT_SINC32;
-    const bool inv = p->sgn > 0;
-    const int L = log2_exact(m2);
-    const int grid_az = std::max(8, std::min((p->n + 7) / 8 * 8, ctx->num_cu / 8 * 8));
-    const TanBins tb{p->tan_bin, p->tb_lo, p->tb_inv_w, p->tb_bins};
-    pfa_timer_begin(ctx, PFA_TIMER_AZIMUTH);
-#define ARGS (const float2 *)g1t, ldp, p->m, p->n, grid_of(p->KR, p->kr_lin, p->n), p->tan_s64, tb, \
-             grid_of(p->KC, p->kc_lin, p->m), p->hm, p->hn, p->sinc, nullptr, nullptr, tw,          \
-             (unsigned *)(ctx->d_counters + 32), A16, expA, nullptr
-#define K(LL, S, I, F) hipLaunchKernelGGL((k_azimuth_f32<LL, 1, S, I, F>), dim3(grid_az), dim3(az_nb<LL>()), 0, ctx->stream, ARGS)
-#define KF(LL, F)                                 \
-    if (sinc && inv) K(LL, true, true, F);        \
-    else if (sinc) K(LL, true, false, F);         \
-    else if (inv) K(LL, false, true, F);          \
-    else K(LL, false, false, F);
-#define LAUNCH(LL)                                 \
-    case LL:                                       \
-        if (fk == FK_BF16) { KF(LL, FK_BF16) }     \
-        else if (fk == FK_FP16) { KF(LL, FK_FP16) } \
-        else { KF(LL, FK_BF16S) }                  \
-        break;
-    PFA_MFMA_SWITCH(L, LAUNCH)
-#undef LAUNCH
-#undef KF
-#undef K
-#undef ARGS
-    pfa_timer_end(ctx, PFA_TIMER_AZIMUTH);
-    PFA_HIP(ctx, hipGetLastError());
-    return PFA_OK;
-}
 
 // The operand table of (L, precision, direction), built on first use and kept
 // in the context's workspace.
@@ -151,6 +125,46 @@ static const pfam::u16x8 *dft_ops(pfa_ctx *ctx, int L, bool bf, bool inv, int *s
         return nullptr;
     }
     return tab;
+}
+
+int launch_float_azimuth_mfma(pfa_plan *p, const void *g1t, int ldp, unsigned *A16, int *expA,
+                              int fk) {
+    pfa_ctx *ctx = p->ctx;
+    int st = PFA_OK;
+    const int m2 = p->m2;
+    const float2 *tw = twiddles_f32(ctx, m2, &st);
+    if (!tw) return st;
+    const bool sinc = p->flags & PFA_FLOAT_SINC32;
+    const bool inv = p->sgn > 0;
+    const int L = log2_exact(m2);
+    const pfam::u16x8 *ops = dft_ops(ctx, L, fk != FK_FP16, inv, &st);
+    if (!ops) return st;
+    const int grid_az = std::max(8, std::min((p->n + 7) / 8 * 8, ctx->num_cu / 8 * 8));
+    const TanBins tb{p->tan_bin, p->tb_lo, p->tb_inv_w, p->tb_bins};
+    pfa_timer_begin(ctx, PFA_TIMER_AZIMUTH);
+#define ARGS (const float2 *)g1t, ldp, p->m, p->n, grid_of(p->KR, p->kr_lin, p->n), p->tan_s64, tb, \
+             grid_of(p->KC, p->kc_lin, p->m), p->hm, p->hn, p->sinc, nullptr, nullptr, tw,          \
+             (unsigned *)(ctx->d_counters + 32), A16, expA, nullptr, ops
+#define K(LL, S, I, F) hipLaunchKernelGGL((k_azimuth_f32<LL, 1, S, I, F>), dim3(grid_az), dim3(az_nb<LL>()), 0, ctx->stream, ARGS)
+#define KF(LL, F)                                 \
+    if (sinc && inv) K(LL, true, true, F);        \
+    else if (sinc) K(LL, true, false, F);         \
+    else if (inv) K(LL, false, true, F);          \
+    else K(LL, false, false, F);
+#define LAUNCH(LL)                                 \
+    case LL:                                       \
+        if (fk == FK_BF16) { KF(LL, FK_BF16) }     \
+        else if (fk == FK_FP16) { KF(LL, FK_FP16) } \
+        else { KF(LL, FK_BF16S) }                  \
+        break;
+    PFA_MFMA_SWITCH(L, LAUNCH)
+#undef LAUNCH
+#undef KF
+#undef K
+#undef ARGS
+    pfa_timer_end(ctx, PFA_TIMER_AZIMUTH);
+    PFA_HIP(ctx, hipGetLastError());
+    return PFA_OK;
 }
 
 int launch_float_fft_b_mfma(pfa_plan *p, const unsigned *A16, const int *expA, void *out,
