@@ -68,8 +68,8 @@ enum { PFA_TIMER_RANGE = 0, PFA_TIMER_AZIMUTH = 1, PFA_TIMER_FFT_B = 2,
 
 /* Diagnostic counters since the last reset: [0] range-pass sinc outputs whose
  * 32-tap window left the staged strip (read from global memory instead), [1]
- * azimuth-pass outputs whose bracket needed the exact fix-up walk beyond the
- * table guess. */
+ * azimuth-pass brackets started from the table guess (the first of each four
+ * consecutive outputs) that needed the exact fix-up walk beyond one step. */
 int pfa_fallback_counts(pfa_ctx *ctx, uint32_t out[2], int reset);
 
 /* ------------------------------------------------------------------ tables */

@@ -320,11 +320,16 @@ __global__ __launch_bounds__(RT, 2) void k_range_f32(
 #pragma unroll
             for (int k = 0; k < RLD; ++k) pre[k] = ld_stream(srow + sdir * min(su + 32 * k, ulast));
         }
-        // brackets of this thread's bins: (c << 8) | phase and the fraction
+        // brackets of this thread's bins: (c << 8) | phase and the fraction.
+        // The first of four consecutive bins starts from the closed-form guess;
+        // the next three walk up from their predecessor's bracket (KR is
+        // non-decreasing), then take the same near-tie re-decision.
         int code[8];
         float frq[8];
 #pragma unroll
         for (int g = 0; g < 2; ++g) {
+            int c = 0;
+            double xl = -INFINITY, xh = INFINITY;
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int k = g * 4 + u;
@@ -333,8 +338,18 @@ __global__ __launch_bounds__(RT, 2) void k_range_f32(
                 frq[k] = 0.f;
                 if (p >= m || q >= qe) continue;
                 const double Q = Qv[k];
-                double xl, xh;
-                const int c = bracket(Q, xl, xh);
+                if (u == 0) {
+                    c = bracket(Q, xl, xh);
+                } else {
+                    int steps = 0;
+                    while (c < n && xh <= Q && steps < 8) { ++c; ++steps; xl = xh; xh = c < n ? xa(c) : INFINITY; }
+                    if (c < n && xh <= Q) {
+                        c = bracket(Q, xl, xh);
+                    } else {
+                        const double tol = fabs(Q) * 4e-15;
+                        if (fabs(xl - Q) <= tol || fabs(xh - Q) <= tol) c = bracket_from(xe, n, Q, c, xl, xh);
+                    }
+                }
                 int ph = 0;
                 if (c == n) {
                     ph = (Q == xl) ? 1 : 0;           // last sample, or outside
@@ -543,21 +558,24 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
             __syncthreads();
             // (2) brackets: c = #{p : xs[p] <= Q} (np.interp / searchsorted-right),
             // kept as (c << 8) | sinc phase, and the interval fraction.  The
-            // tan-bin guesses and the window's taper values are loaded for all
-            // outputs first, so their global round trips overlap.
-            int gix[G * 4];
+            // first output of each group of four starts from the tan-bin guess;
+            // the next three walk up from their predecessor's bracket (KC is
+            // non-decreasing).  Guesses and the window's taper values are
+            // loaded before any is used, so their global round trips overlap.
+            int gix[G];
 #pragma unroll
-            for (int k = 0; k < G * 4; ++k) gix[k] = tb.bin(__dmul_rn(KC(min(((k / 4) * NB + tl) * 4 + k % 4, m - 1)), inv_krj));
+            for (int g = 0; g < G; ++g) gix[g] = tb.bin(__dmul_rn(KC(min((g * NB + tl) * 4, m - 1)), inv_krj));
             double hmv[G * 4];
 #pragma unroll
-            for (int k = 0; k < G * 4; ++k) {
-                gix[k] = tb.idx[gix[k]];
-                hmv[k] = hm ? hm[min(((k / 4) * NB + tl) * 4 + k % 4, m - 1)] : 1.0;
-            }
+            for (int g = 0; g < G; ++g) gix[g] = tb.idx[gix[g]];
+#pragma unroll
+            for (int k = 0; k < G * 4; ++k) hmv[k] = hm ? hm[min(((k / 4) * NB + tl) * 4 + k % 4, m - 1)] : 1.0;
             int code[G * 4];
             float frq[G * 4];
 #pragma unroll
             for (int g = 0; g < G; ++g) {
+                int c = 0;
+                double xl = -INFINITY, xh = INFINITY;
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int q = (g * NB + tl) * 4 + u;
@@ -567,17 +585,21 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
                     wl[k * NB + t] = (float)__dmul_rn(hmv[k], hj);
                     if (q >= m) continue;
                     const double Q = KC(q);
-                    int c = gix[k];
-                    double xl = c > 0 ? xs[c - 1] : -INFINITY;
-                    double xh = c < m ? xs[c] : INFINITY;
-                    if (xh <= Q) {                    // the guess is low by one for ~1/4 of outputs
-                        ++c;
-                        xl = xh;
+                    if (u == 0) {
+                        c = gix[g];
+                        xl = c > 0 ? xs[c - 1] : -INFINITY;
                         xh = c < m ? xs[c] : INFINITY;
-                    }
-                    if (!(xl <= Q) || xh <= Q) {
-                        ++slow;
-                        while (c > 0 && !(xl <= Q)) { --c; xh = xl; xl = c > 0 ? xs[c - 1] : -INFINITY; }
+                        if (xh <= Q) {                // the guess is low by one for ~1/4 of outputs
+                            ++c;
+                            xl = xh;
+                            xh = c < m ? xs[c] : INFINITY;
+                        }
+                        if (!(xl <= Q) || xh <= Q) {
+                            ++slow;
+                            while (c > 0 && !(xl <= Q)) { --c; xh = xl; xl = c > 0 ? xs[c - 1] : -INFINITY; }
+                            while (c < m && xh <= Q) { ++c; xl = xh; xh = c < m ? xs[c] : INFINITY; }
+                        }
+                    } else {
                         while (c < m && xh <= Q) { ++c; xl = xh; xh = c < m ? xs[c] : INFINITY; }
                     }
                     int ph = 0;
