@@ -149,6 +149,7 @@ __device__ __forceinline__ void pass(float2 *buf, const Tw &tw, Src *src, Sink &
     // otherwise be hoisted and held in registers across the whole loop
     int t = threadIdx.x;
     asm volatile("" : "+v"(t));
+    t &= 1023;                                         // known non-negative: k / LS, k % LS are shifts
     float2 x[B][R];
     if (t < NT) {
 #pragma unroll

@@ -218,6 +218,8 @@ __global__ __launch_bounds__(RT, 2) void k_range_f32(
     __shared__ int lo_s[RP], hi_s[RP], st_s[RP], dir_s[RP];
     __shared__ int ps_i[RP];                          // per pulse of the current group: sorted index
     __shared__ double ps_a[RP], ps_f0[RP], ps_df[RP], ps_pr[RP];  // 2 pr / c, f0, df, pr
+    __shared__ double ps_x0[RP], ps_iad[RP];          // a f0, 1 / (a df) (bracket guesses)
+    __shared__ int ps_asc[RP];                        // ascending positions
     auto tile = reinterpret_cast<float2 (*)[RP + 1]>(&strip[0]);
     if (SINC) load_taps2(tp, sinc);
     const int tiles_p = (m + RP - 1) / RP, tiles_q = (n + RQ - 1) / RQ;
@@ -228,6 +230,7 @@ __global__ __launch_bounds__(RT, 2) void k_range_f32(
         const int qe = min(q0 + RQ, n);
         int t = threadIdx.x;                          // opaque: per-thread values stay in the loop
         asm volatile("" : "+v"(t));
+        t &= RT - 1;                                  // ... known in [0, RT): shifts, not signed division
         const int pl = t % RP, gi = t / RP;           // pulse in tile, bin group
         const int p = p0 + pl;
         if (q0 == 0) {                                // first tile of a pulse group: its pulse values to LDS
@@ -252,6 +255,11 @@ __global__ __launch_bounds__(RT, 2) void k_range_f32(
                 ps_f0[t] = ff;
                 ps_df[t] = dd;
                 ps_pr[t] = pv;
+                auto fqj = [&](int j) { return FULLFREQ ? freq[(size_t)ii * n + j] : __dadd_rn(ff, __dmul_rn((double)j, dd)); };
+                auto knp = [&](int j) { return __dmul_rn(__ddiv_rn(2.0 * fqj(j), C_LIGHT), pv); };
+                ps_asc[t] = p0 + t < m ? !(knp(0) > knp(n - 1)) : 1;
+                ps_x0[t] = aa * ff;
+                ps_iad[t] = 1.0 / (aa * dd);
             }
             __syncthreads();
         }
@@ -273,10 +281,10 @@ __global__ __launch_bounds__(RT, 2) void k_range_f32(
         auto kr = [&](int j) { return __dmul_rn(ai, fq(j)); };
         const double prv = ps_pr[pl];
         auto kr_np = [&](int j) { return __dmul_rn(__ddiv_rn(2.0 * fq(j), C_LIGHT), prv); };
-        const bool asc = p < m ? !(kr_np(0) > kr_np(n - 1)) : true;
+        const bool asc = ps_asc[pl] != 0;
         auto xa = [&](int u) { return kr(asc ? u : n - 1 - u); };
         auto xe = [&](int u) { return kr_np(asc ? u : n - 1 - u); };
-        const double x0 = ai * f0, inv_adf = 1.0 / (ai * df);
+        const double x0 = ps_x0[pl], inv_adf = ps_iad[pl];
         auto guess = [&](double Q) {                  // ascending-view count estimate
             const double js = (Q - x0) * inv_adf;
             const double g = asc ? js : (double)(n - 1) - js;
@@ -423,9 +431,16 @@ __global__ __launch_bounds__(RT, 2) void k_range_f32(
 #pragma unroll
             for (int u = 0; u < 4; ++u) tile[4 * (gi + g * (RT / RP)) + u][pl] = v[g * 4 + u];
         __syncthreads();
-        for (int r = t; r < RQ * RP; r += RT) {
-            const int qq = r / RP, pp = r % RP;
-            if (q0 + qq < n && p0 + pp < m) st_stream(g1t + (size_t)(q0 + qq) * ldp + p0 + pp, tile[qq][pp]);
+        {                                             // all tile reads first, then the stores
+            constexpr int NS = RQ * RP / RT, QS = RT / RP;
+            const int qq = t / RP, pp = t % RP;
+            float2 w[NS];
+#pragma unroll
+            for (int i = 0; i < NS; ++i) w[i] = tile[qq + i * QS][pp];
+            float2 *dst = g1t + (size_t)(q0 + qq) * ldp + p0 + pp;
+#pragma unroll
+            for (int i = 0; i < NS; ++i)
+                if (q0 + qq + i * QS < n && p0 + pp < m) st_stream(dst + (size_t)(i * QS) * ldp, w[i]);
         }
     }
     count_wave(nslow, slow);
