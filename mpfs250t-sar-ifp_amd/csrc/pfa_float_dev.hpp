@@ -106,7 +106,7 @@ __device__ __forceinline__ void sinc4p(const float2 *row, int start, const unsig
         ai[u] = 0.f;
     }
     const float2 *ws = row + start;
-#pragma unroll 1
+#pragma unroll
     for (int kb = 0; kb < KW; kb += BATCH) {
         const float2 *wa = ws, *wb = ws;               // before / after the spare slot
         int kbrk = BATCH;
