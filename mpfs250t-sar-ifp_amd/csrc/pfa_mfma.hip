@@ -42,24 +42,16 @@ __global__ __launch_bounds__(fftb_mfma_nb<L>()) void k_fft_b_mfma(
     const pfa32::Tw tw = pfa32::load_tw<N2>(twl, twg);
     const int t = threadIdx.x;
     const int r0 = xcd_map(blockIdx.x, m2 / NCOL) * NCOL;
-    // common exponent of the block: |A16| < 2^2, so 2^-(max expA + 2) bounds every input by 1
-    int e = INT_MIN;
-    for (int jj = t; jj < n; jj += NB) e = max(e, expA[jj]);
-    for (int off = 32; off >= 1; off >>= 1) e = max(e, __shfl_xor(e, off, 64));
-    if ((t & 63) == 0) emx[t >> 6] = e;
-    __syncthreads();
-    int S = INT_MIN;
-#pragma unroll
-    for (int i = 0; i < BW; ++i) S = max(S, emx[i]);
-    S = (S == INT_MIN) ? 0 : S + 2;
-    // the corner-turn: 4 adjacent cross bins of every range row, rescaled to
-    // 2^-S.  Lane t takes row k + r NV0 (r = t % R0): consecutive lanes write
-    // consecutive words of the first-stage layout (rows are 4 m2 bytes apart in
-    // A16, so the assignment does not change the global requests).
+    // the corner-turn: 4 adjacent cross bins of every range row.  Lane t takes
+    // row k + r NV0 (r = t % R0): consecutive lanes write consecutive words of
+    // the first-stage layout (rows are 4 m2 bytes apart in A16, so the
+    // assignment does not change the global requests).  All loads are issued
+    // first (clamped rows); the row exponents loaded with them also give the
+    // block's common exponent.
     constexpr int R0 = pfam::MPlan<L>::R[0], NV0 = N2 / R0, NI = N2 / NB;
     static_assert(NB % R0 == 0 && N2 % NB == 0, "row assignment covers the line");
     auto row_of = [&](int i) { return t / R0 + i * (NB / R0) + (t % R0) * NV0; };
-    uint4 a[NI];                                       // all loads in flight first (clamped rows)
+    uint4 a[NI];
     int ex[NI];
 #pragma unroll
     for (int i = 0; i < NI; ++i) {
@@ -67,6 +59,17 @@ __global__ __launch_bounds__(fftb_mfma_nb<L>()) void k_fft_b_mfma(
         a[i] = *reinterpret_cast<const uint4 *>(A16 + (size_t)jc * m2 + r0);
         ex[i] = expA[jc];
     }
+    // common exponent: |A16| < 2^2, so 2^-(max expA + 2) bounds every input by 1
+    int e = INT_MIN;
+#pragma unroll
+    for (int i = 0; i < NI; ++i) e = max(e, ex[i]);  // clamped rows repeat row n - 1
+    for (int off = 32; off >= 1; off >>= 1) e = max(e, __shfl_xor(e, off, 64));
+    if ((t & 63) == 0) emx[t >> 6] = e;
+    __syncthreads();
+    int S = INT_MIN;
+#pragma unroll
+    for (int i = 0; i < BW; ++i) S = max(S, emx[i]);
+    S = (S == INT_MIN) ? 0 : S + 2;
 #pragma unroll
     for (int i = 0; i < NI; ++i) {
         const int jj = row_of(i);
