@@ -102,6 +102,7 @@ extern "C" void pfa_destroy(pfa_ctx *ctx) {
     if (!ctx) return;
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
+    pfa_hostio_destroy(ctx);
     for (auto &kv : ctx->ws)
         if (kv.second.ptr) hipFree(kv.second.ptr);
     for (auto &mk : ctx->marks) {
@@ -233,10 +234,9 @@ extern "C" int pfa_quantize_iq16(pfa_ctx *ctx, const float *sig, int64_t count, 
     if (!d_in) return st;
     int16_t *d_out = (int16_t *)pfa_ws(ctx, "io_out", out_b, &st);
     if (!d_out) return st;
-    PFA_HIP(ctx, hipMemcpyAsync(d_in, sig, in_b, hipMemcpyHostToDevice, ctx->stream));
+    CK(pfa_io_h2d(ctx, d_in, sig, in_b));
     CK(launch_quantize(ctx, d_in, count * 2, d_out, exp_out));
-    PFA_HIP(ctx, hipMemcpyAsync(out, d_out, out_b, hipMemcpyDeviceToHost, ctx->stream));
-    PFA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    CK(pfa_io_d2h(ctx, out, d_out, out_b));
     return PFA_OK;
 }
 
@@ -273,11 +273,10 @@ static int fft_rows_host(pfa_ctx *ctx, const int16_t *x, int rows, int n, int16_
     if (!dy) return st;
     int32_t *de = (int32_t *)pfa_ws(ctx, "io_e", (size_t)rows * 4, &st);
     if (!de) return st;
-    PFA_HIP(ctx, hipMemcpyAsync(dx, x, b, hipMemcpyHostToDevice, ctx->stream));
+    CK(pfa_io_h2d(ctx, dx, x, b));
     CK(fft_rows_dev(ctx, (const int16_t *)dx, rows, n, (int16_t *)dy, de, renorm));
-    PFA_HIP(ctx, hipMemcpyAsync(y, dy, b, hipMemcpyDeviceToHost, ctx->stream));
     PFA_HIP(ctx, hipMemcpyAsync(e, de, (size_t)rows * 4, hipMemcpyDeviceToHost, ctx->stream));
-    PFA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    CK(pfa_io_d2h(ctx, y, dy, b));
     return PFA_OK;
 }
 
@@ -309,12 +308,11 @@ extern "C" int pfa_window_fixed(pfa_ctx *ctx, const int16_t *g2, int rows, int c
     if (!dy) return st;
     int32_t *dh = (int32_t *)pfa_ws(ctx, "io_ham", (size_t)(rows + cols) * 4, &st);
     if (!dh) return st;
-    PFA_HIP(ctx, hipMemcpyAsync(dx, g2, b, hipMemcpyHostToDevice, ctx->stream));
-    PFA_HIP(ctx, hipMemcpyAsync(dh, hamr, (size_t)rows * 4, hipMemcpyHostToDevice, ctx->stream));
-    PFA_HIP(ctx, hipMemcpyAsync(dh + rows, hamc, (size_t)cols * 4, hipMemcpyHostToDevice, ctx->stream));
+    CK(pfa_io_h2d(ctx, dx, g2, b));
+    CK(pfa_io_h2d(ctx, dh, hamr, (size_t)rows * 4));
+    CK(pfa_io_h2d(ctx, dh + rows, hamc, (size_t)cols * 4));
     CK(launch_window_fixed(ctx, dx, rows, cols, dh, dh + rows, dy));
-    PFA_HIP(ctx, hipMemcpyAsync(out, dy, b, hipMemcpyDeviceToHost, ctx->stream));
-    PFA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    CK(pfa_io_d2h(ctx, out, dy, b));
     return PFA_OK;
 }
 
@@ -327,10 +325,9 @@ extern "C" int pfa_detect_fixed(pfa_ctx *ctx, const int16_t *z, int64_t count, u
     if (!dz) return st;
     void *dy = pfa_ws(ctx, "io_out", (size_t)count * 2, &st);
     if (!dy) return st;
-    PFA_HIP(ctx, hipMemcpyAsync(dz, z, (size_t)count * 4, hipMemcpyHostToDevice, ctx->stream));
+    CK(pfa_io_h2d(ctx, dz, z, (size_t)count * 4));
     CK(launch_detect_fixed(ctx, dz, count, (uint16_t *)dy));
-    PFA_HIP(ctx, hipMemcpyAsync(out, dy, (size_t)count * 2, hipMemcpyDeviceToHost, ctx->stream));
-    PFA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    CK(pfa_io_d2h(ctx, out, dy, (size_t)count * 2));
     return PFA_OK;
 }
 
@@ -433,10 +430,9 @@ extern "C" int pfa_resample_fixed(pfa_plan *p, const int16_t *sig, int16_t *g2) 
     if (!ds) return st;
     void *dg = pfa_ws(ctx, "io_out", out_b, &st);
     if (!dg) return st;
-    PFA_HIP(ctx, hipMemcpyAsync(ds, sig, in_b, hipMemcpyHostToDevice, ctx->stream));
+    CK(pfa_io_h2d(ctx, ds, sig, in_b));
     CK(resample_fixed_dev(p, ds, dg));
-    PFA_HIP(ctx, hipMemcpyAsync(g2, dg, out_b, hipMemcpyDeviceToHost, ctx->stream));
-    PFA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    CK(pfa_io_d2h(ctx, g2, dg, out_b));
     return PFA_OK;
 }
 
@@ -479,12 +475,11 @@ extern "C" int pfa_form_image_fixed(pfa_plan *p, const int16_t *sig, uint16_t *o
     if (!dout) return st;
     int32_t *de = (int32_t *)pfa_ws(ctx, "io_e", (size_t)G * 8, &st);
     if (!de) return st;
-    PFA_HIP(ctx, hipMemcpyAsync(ds, sig, in_b, hipMemcpyHostToDevice, ctx->stream));
+    CK(pfa_io_h2d(ctx, ds, sig, in_b));
     CK(pfa_form_image_fixed_dev(p, (const int16_t *)ds, (uint16_t *)dout, de, de + G));
-    PFA_HIP(ctx, hipMemcpyAsync(out, dout, out_b, hipMemcpyDeviceToHost, ctx->stream));
     if (exp_a) PFA_HIP(ctx, hipMemcpyAsync(exp_a, de, (size_t)G * 4, hipMemcpyDeviceToHost, ctx->stream));
     if (exp_b) PFA_HIP(ctx, hipMemcpyAsync(exp_b, de + G, (size_t)G * 4, hipMemcpyDeviceToHost, ctx->stream));
-    PFA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    CK(pfa_io_d2h(ctx, out, dout, out_b));
     return PFA_OK;
 }
 
@@ -719,12 +714,11 @@ extern "C" int pfa_focus_f32(pfa_plan *p, const float *sig, const double *freq, 
     if (freq) {
         dfreq = (double *)pfa_ws(ctx, "io_freq", (size_t)p->m * p->n * 8, &st);
         if (!dfreq) return st;
-        PFA_HIP(ctx, hipMemcpyAsync(dfreq, freq, (size_t)p->m * p->n * 8, hipMemcpyHostToDevice, ctx->stream));
+        CK(pfa_io_h2d(ctx, dfreq, freq, (size_t)p->m * p->n * 8));
     }
-    PFA_HIP(ctx, hipMemcpyAsync(ds, sig, in_b, hipMemcpyHostToDevice, ctx->stream));
+    CK(pfa_io_h2d(ctx, ds, sig, in_b));
     CK(pfa_focus_f32_dev(p, (const float *)ds, dfreq, dout, out_kind));
-    PFA_HIP(ctx, hipMemcpyAsync(out, dout, out_b, hipMemcpyDeviceToHost, ctx->stream));
-    PFA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    CK(pfa_io_d2h(ctx, out, dout, out_b));
     return PFA_OK;
 }
 
@@ -742,16 +736,15 @@ extern "C" int pfa_resample_kspace_f32(pfa_plan *p, const float *sig, const doub
     if (freq) {
         dfreq = (double *)pfa_ws(ctx, "io_freq", (size_t)p->m * p->n * 8, &st);
         if (!dfreq) return st;
-        PFA_HIP(ctx, hipMemcpyAsync(dfreq, freq, (size_t)p->m * p->n * 8, hipMemcpyHostToDevice, ctx->stream));
+        CK(pfa_io_h2d(ctx, dfreq, freq, (size_t)p->m * p->n * 8));
     }
     const int ldp = ldp32(p->m);
     void *g1t = pfa_ws(ctx, "f_g1t", (size_t)p->n * ldp * 8, &st);
     if (!g1t) return st;
-    PFA_HIP(ctx, hipMemcpyAsync(ds, sig, in_b, hipMemcpyHostToDevice, ctx->stream));
+    CK(pfa_io_h2d(ctx, ds, sig, in_b));
     CK(launch_float_range(p, ds, dfreq, g1t, ldp));
     CK(launch_float_azimuth(p, 0, g1t, ldp, nullptr, dg));
-    PFA_HIP(ctx, hipMemcpyAsync(g2, dg, in_b, hipMemcpyDeviceToHost, ctx->stream));
-    PFA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    CK(pfa_io_d2h(ctx, g2, dg, in_b));
     return PFA_OK;
 }
 
@@ -769,10 +762,9 @@ extern "C" int pfa_transform_f32(pfa_plan *p, const float *g2, void *out, int ou
     if (!dout) return st;
     void *A = pfa_ws(ctx, "f_A", (size_t)(p->n + 15) / 16 * 16 * p->m2 * 8, &st);
     if (!A) return st;
-    PFA_HIP(ctx, hipMemcpyAsync(dg, g2, in_b, hipMemcpyHostToDevice, ctx->stream));
+    CK(pfa_io_h2d(ctx, dg, g2, in_b));
     CK(launch_float_azimuth(p, 2, nullptr, 0, dg, A));
     CK(launch_float_fft_b(p, A, dout, out_kind));
-    PFA_HIP(ctx, hipMemcpyAsync(out, dout, out_b, hipMemcpyDeviceToHost, ctx->stream));
-    PFA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    CK(pfa_io_d2h(ctx, out, dout, out_b));
     return PFA_OK;
 }

@@ -21,6 +21,8 @@ struct TimerMark {
     hipEvent_t start, stop;
 };
 
+struct pfa_hostio;                       // staged host copies (pfa_hostio.cpp)
+
 struct pfa_ctx {
     int device = 0;
     int num_cu = 256;                     // compute units of the device
@@ -33,6 +35,8 @@ struct pfa_ctx {
     std::vector<hipEvent_t> event_pool;
     int *d_counters = nullptr;            // [0]=emax pass A, [1]=emax pass B, [2..] scratch
     unsigned *d_maxbits = nullptr;        // quantiser max |x| as float bits
+    pfa_hostio *io = nullptr;             // created on the first large host copy
+    bool io_failed = false;               // staging unavailable: copies go direct
 };
 
 enum PlanKind { PLAN_FIXED = 1, PLAN_FLOAT = 2 };
@@ -69,6 +73,12 @@ int pfa_hip_fail(pfa_ctx *ctx, hipError_t e, const char *what);
 void *pfa_ws(pfa_ctx *ctx, const char *slot, size_t bytes, int *status);
 void pfa_timer_begin(pfa_ctx *ctx, int kind);
 void pfa_timer_end(pfa_ctx *ctx, int kind);
+// Host <-> device copies of the host-buffer entry points: large ones staged
+// through pinned buffers on a copy stream (pfa_hostio.cpp).  h2d leaves the
+// context stream waiting for the data; d2h returns when the host has it.
+int pfa_io_h2d(pfa_ctx *ctx, void *dst, const void *src, size_t bytes);
+int pfa_io_d2h(pfa_ctx *ctx, void *dst, const void *src, size_t bytes);
+void pfa_hostio_destroy(pfa_ctx *ctx);
 
 #define PFA_HIP(ctx, call)                                        \
     do {                                                          \
