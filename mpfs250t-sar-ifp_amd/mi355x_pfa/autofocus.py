@@ -117,5 +117,5 @@ def autofocus(plan, steps=50, lr=0.05, phi0=None, record=False):
         s = sharpness(phi, plan)
         (-s).backward()
         opt.step()
-        hist.append(float(s))
+        hist.append(float(s.detach()))
     return phi.detach(), np.array(hist)
