@@ -127,11 +127,9 @@ def time_fixed(sc, ax, ay, geom, steps, warmup, torch):
     return wall, kern
 
 
-def precision_forms(spec, sc, geom, interp, forms, steps, warmup, torch):
-    """Time each FFT form on the frame and state its error against an FP64
-    reference: the centred 2-D FFT in double precision (on the GPU) of this
-    frame's FP32 k-space (the resample stage every form shares)."""
-    from mi355x_pfa import metrics
+def _fp64_reference(spec, sc, geom, interp, torch):
+    """Centred 2-D FFT in double precision (on the GPU) of the frame's FP32
+    k-space (the resample stage every FFT form shares)."""
     from mi355x_pfa.pfa import FloatPlan
     m, n = spec.pulses, spec.samples
     base = FloatPlan(m, n, -1, geom, f0=sc.sc0, df=sc.scss, window=True, interp=interp)
@@ -143,6 +141,20 @@ def precision_forms(spec, sc, geom, interp, forms, steps, warmup, torch):
     del g2
     ref = torch.fft.fftshift(torch.fft.fft2(torch.fft.ifftshift(pad)))
     del pad
+    return ref
+
+
+def precision_forms(spec, sc, geom, interp, forms, steps, warmup, torch):
+    """Time each FFT form on the frame and state its error against an FP64
+    reference of the same FP32 k-space: RMS and peak positions on this frame,
+    PSLR / ISLR on a one-target frame of the same geometry without clutter (in
+    the cluttered frame the side-lobe region holds other scatterers)."""
+    import dataclasses
+    from mi355x_pfa import metrics
+    from mi355x_pfa.pfa import FloatPlan
+    m, n = spec.pulses, spec.samples
+    ref = _fp64_reference(spec, sc, geom, interp, torch)
+    m2, n2 = ref.shape
     ref_abs = ref.abs()
     peak = float(ref_abs.max())
     ref_host = ref_abs.float().cpu().numpy()
@@ -157,21 +169,36 @@ def precision_forms(spec, sc, geom, interp, forms, steps, warmup, torch):
         torch.cuda.synchronize()
         err = float(torch.sqrt(torch.mean((img.to(torch.complex128) - ref).abs() ** 2)) / peak)
         mag = img.abs().cpu().numpy()
-        pk = metrics.top_peaks(ref_host, 1)[0]
-        ps, isl = metrics.pslr_islr(mag, pk)
-        ps_r, isl_r = metrics.pslr_islr(ref_host, pk)
         bk = algorithmic_bytes_float(m, n, m2, n2, s_a=8 if form == "fp32" else 4)
         out[form] = {"ms_per_frame": round(wall, 4),
                      "kernels_ms": {k: round(v, 4) for k, v in kern.items()},
                      "kernels_gbps": {k: round(bk[k] / (kern[k] * 1e-3) / 1e9, 1) for k in kern},
                      "error_vs_fp64": {"pn_rms": float(f"{err:.3e}"),
-                                       "peak_shift_px": metrics.peak_shift(mag, ref_host, 8),
-                                       "pslr_db": round(ps, 3), "islr_db": round(isl, 3),
-                                       "pslr_fp64_db": round(ps_r, 3),
-                                       "islr_fp64_db": round(isl_r, 3)}}
+                                       "peak_shift_px": metrics.peak_shift(mag, ref_host, 8)}}
         del det, img, mag
         plan.close()
-    del ref, ref_abs
+    del ref, ref_abs, ref_host
+    torch.cuda.empty_cache()
+    # point-target frame for the side-lobe figures
+    spec1 = dataclasses.replace(spec, targets=1, clutter=0, scene_half=50.0)
+    sc1, _, _, geom1 = make_inputs(spec1, "cuda")
+    ref1 = _fp64_reference(spec1, sc1, geom1, interp, torch).abs().float().cpu().numpy()
+    pk = metrics.top_peaks(ref1, 1)[0]
+    ps_r, isl_r = metrics.pslr_islr(ref1, pk)
+    for form in forms:
+        plan = FloatPlan(m, n, -1, geom1, f0=sc1.sc0, df=sc1.scss, window=True, interp=interp,
+                         fft=form)
+        img = torch.empty((m2, n2), dtype=torch.float32, device="cuda")
+        plan.focus_dev(sc1.sig, img, detect=True)
+        torch.cuda.synchronize()
+        ps, isl = metrics.pslr_islr(img.cpu().numpy(), pk)
+        out[form]["error_vs_fp64"].update({
+            "pslr_db": round(ps, 3), "islr_db": round(isl, 3),
+            "pslr_fp64_db": round(ps_r, 3), "islr_fp64_db": round(isl_r, 3),
+            "sidelobe_frame": "1 point target, no clutter, same geometry and shape"})
+        del img
+        plan.close()
+    del sc1, ref1
     torch.cuda.empty_cache()
     return out
 

@@ -488,7 +488,8 @@ struct TanBins {
 // Thread t owns outputs 4 (g NB + t) + u, u < 4: the four share one sinc window.
 // MODE 0: store the windowed k-space g2[q][j] (resample_kspace stage output)
 // MODE 1: fused -- window, sign fold, pad, FFT along cross -> A (row-major)
-// MODE 2: FFT along cross of a given g2 (focus() stage) -> A (row-major)
+// MODE 2: FFT along cross of a given g2 (focus() stage; windowed when the plan has tapers:
+//         quicklook()) -> A (row-major)
 // FFT form of the line transform: 0 FP32 vector FFT (A complex64); 1 BF16,
 // 2 FP16, 3 BF16 with hi/lo DFT matrices -- matrix-core FFT (pfa_fft_mfma.hpp),
 // A stored as packed 16-bit pairs with a per-row exponent.
@@ -551,9 +552,16 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
         asm volatile("" : "+v"(tl));
         float2 v[G * 4];
         if (MODE == 2) {
+            // a given k-space (focus()), or with the plan's tapers the raw
+            // history of quicklook(): f32(hm[q] hn[j]) as hamming2d forms it
+            const double hj = hn ? hn[j] : 1.0;
             __syncthreads();                          // previous line fully consumed
             for (int q = t; q < M2; q += NB) {
                 float2 w = q < m ? g2in[(size_t)q * n + j] : make_float2(0.f, 0.f);
+                if (hm && q < m) {
+                    const float wgt = (float)__dmul_rn(hm[q], hj);
+                    w = make_float2(w.x * wgt, w.y * wgt);
+                }
                 if ((q + j) & 1) w = make_float2(-w.x, -w.y);
                 reg[pfa32::padf(q)] = w;
             }

@@ -7,6 +7,7 @@ TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
                           reference's interpolator) plus the SINC32 variant
 * ``focus``            -- form_image_pfa.py:225-240 with transform2d :144-147
 * ``pfa``              -- form_image_pfa.py:243-246
+* ``quicklook``        -- form_image_pfa.py:154-162 (hamming2d + transform2d)
 
 ``store`` selects the intermediate precision: complex64 reproduces the
 reference; complex128 gives the float64 oracle that the reduced-precision
@@ -150,6 +151,17 @@ def pfa(sig, freq, ax, ay, sgn, window=True, interp="linear", store=np.complex64
     """form_image_pfa.py:243-246."""
     g2, geo = resample_kspace(sig, freq, ax, ay, window=window, interp=interp, store=store)
     return focus(g2, geo, sgn)
+
+
+def quicklook(sig, sgn, window=True):
+    """form_image_pfa.py:158-162: the raw history times hamming2d (:154-155, the
+    float32 outer product of np.hamming) when window is set, then transform2d
+    (:144-147) with no padding."""
+    sig = np.asarray(sig)
+    if window:
+        sig = sig * np.outer(np.hamming(sig.shape[0]), np.hamming(sig.shape[1])).astype(np.float32)
+    fwd = np.fft.fft2 if sgn < 0 else np.fft.ifft2
+    return np.fft.fftshift(fwd(np.fft.ifftshift(sig)))
 
 
 def detect(img):

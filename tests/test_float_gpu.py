@@ -111,6 +111,24 @@ def test_compact_frequencies_equal_full_array(gp):
     assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("m,n,sgn,window", [(256, 512, -1, True), (1024, 1024, 1, True),
+                                            (512, 256, -1, False)])
+def test_quicklook_matches_oracle(gp, m, n, sgn, window):
+    """form_image_pfa.quicklook (window + one centred 2-D FFT of the raw history)."""
+    sc, freq, ax, ay = _scene(m, n, seed=m + n)
+    img = gp.quicklook(sc.sig, sgn, window=window)
+    ref = rfl.quicklook(sc.sig, sgn, window=window)
+    assert img.shape == (m, n)
+    assert pn_rms(img, ref) <= TOL
+    assert top_peaks(img, 4) == top_peaks(ref, 4)
+
+
+def test_quicklook_refuses_other_sizes(gp):
+    from mi355x_pfa._lib import PfaError
+    with pytest.raises(PfaError, match="powers of two"):
+        gp.quicklook(np.zeros((300, 256), np.complex64), -1)
+
+
 def test_host_buffer_path_equals_device_path(gp):
     """The host-buffer entry points stage large copies through a ring of pinned
     buffers on a copy stream: 18.5 MB of signal and of frequencies in (two chunks,

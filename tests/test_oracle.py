@@ -161,6 +161,29 @@ def test_float_plain_fft_special_case_and_parseval():
     assert abs(np.sum(np.abs(img.astype(np.complex128)) ** 2) - e_in) / e_in < 1e-5
 
 
+def test_quicklook_oracle_pinned():
+    """quicklook = hamming2d x history, then transform2d without padding.  On
+    power-of-two shapes it equals focus() (pinned to the reference fixtures) of
+    the same array, windowed or not; a tone at DFT bin (a, b) becomes a single
+    pixel at the centred bin, with sgn selecting fft2 / ifft2."""
+    rs = np.random.RandomState(3)
+    x = (rs.randn(64, 128) + 1j * rs.randn(64, 128)).astype(np.complex64)
+    for sgn in (-1, 1):
+        assert np.array_equal(rfl.quicklook(x, sgn, window=False), rfl.focus(x, {"dc": 1, "dr": 1}, sgn)[0])
+        w = np.outer(np.hamming(64), np.hamming(128)).astype(np.float32)
+        assert np.array_equal(rfl.quicklook(x, sgn), rfl.focus(x * w, {"dc": 1, "dr": 1}, sgn)[0])
+    m, n, a, b = 64, 128, 5, 17
+    k = np.arange(m)[:, None] - m // 2
+    c = np.arange(n)[None, :] - n // 2
+    tone = np.exp(2j * np.pi * (a * k / m + b * c / n))
+    img = np.abs(rfl.quicklook(tone, 1, window=False))       # ifft2 (1/(m n)): peak at -(a, b)
+    assert np.unravel_index(np.argmax(img), img.shape) == (m // 2 - a, n // 2 - b)
+    assert np.isclose(img.max(), 1.0) and np.sort(img.ravel())[-2] < 1e-9
+    img = np.abs(rfl.quicklook(tone, -1, window=False))      # fft2: peak m n at +(a, b)
+    assert np.unravel_index(np.argmax(img), img.shape) == (m // 2 + a, n // 2 + b)
+    assert np.isclose(img.max(), m * n)
+
+
 def test_sinc32_float_consistent_with_fixed_sinc():
     """Float SINC32 on integer data equals the reference-pinned fixed gather
     rounded half-up, wherever the two brackets agree (no exact hits)."""
