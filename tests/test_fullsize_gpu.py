@@ -1,0 +1,112 @@
+"""The float focus at the benchmark's full size, 8167 x 8192 -> 8192^2, checked
+through properties that do not need an oracle run of that size (the numpy
+oracle takes minutes per frame): linearity of the whole chain, point targets at
+the pixels the geometry predicts, the matrix-core FFT forms within their bound
+of the FP32 form, and the detected output equal to |complex output|.
+
+Tolerances: linearity to 1e-6 of the peak (FP32 rounding of three frames);
+matrix-core forms to 1e-3 peak-normalised RMS of the FP32 image (the C4 bound);
+target peaks within 1 pixel of the prediction."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def frame():
+    import torch
+    from mi355x_pfa import synth
+    from mi355x_pfa.pfa import float_geometry
+    # targets within 40 m of the scene centre: the polar format's own geometric
+    # distortion, about r^2 / (2 R) at 10 km standoff, stays below 0.2 pixel
+    spec = dataclasses.replace(synth.C3, targets=8, clutter=0, scene_half=40.0, seed=11)
+    print(f"full-size frame {spec.pulses}x{spec.samples}", flush=True)
+    sc = synth.make_scene(spec, device="cuda")
+    ax, ay = sc.look()
+    geom = float_geometry(ax, ay, f0=sc.sc0, df=sc.scss, n=spec.samples)
+    torch.cuda.synchronize()
+    return spec, sc, geom
+
+
+def _plan(frame, fft="fp32", interp="sinc32"):
+    from mi355x_pfa.pfa import FloatPlan
+    spec, sc, geom = frame
+    return FloatPlan(spec.pulses, spec.samples, -1, geom, f0=sc.sc0, df=sc.scss, window=True,
+                     interp=interp, fft=fft)
+
+
+def _focus(plan, sig, detect=False):
+    import torch
+    out = torch.empty((plan.m2, plan.n2), dtype=torch.float32 if detect else torch.complex64,
+                      device="cuda")
+    torch.cuda.synchronize()                      # inputs written on torch's stream
+    plan.focus_dev(sig, out, detect=detect)
+    plan.ctx.synchronize()
+    return out
+
+
+def test_fullsize_linearity(frame):
+    import torch
+    spec, sc, geom = frame
+    plan = _plan(frame)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    b = torch.complex(torch.randn(sc.sig.shape, device="cuda", generator=g),
+                      torch.randn(sc.sig.shape, device="cuda", generator=g)) * 0.1
+    ia = _focus(plan, sc.sig)
+    ib = _focus(plan, b)
+    ic = _focus(plan, (sc.sig + 2 * b).contiguous())
+    peak = float(ic.abs().max())
+    err = float((ic - (ia + 2 * ib)).abs().max()) / peak
+    print(f"linearity: max |I(a + 2b) - I(a) - 2 I(b)| / peak = {err:.2e}", flush=True)
+    assert err <= 1e-6
+    plan.close()
+
+
+def test_fullsize_point_targets_at_predicted_pixels(frame):
+    spec, sc, geom = frame
+    plan = _plan(frame)
+    a = _focus(plan, sc.sig, detect=True).cpu().numpy()
+    geo = plan.padded_geo()
+    dh, ch = np.array(geo["dhat"]), np.array(geo["chat"])
+    for p in sc.targets:
+        col = plan.n2 // 2 - (p[:2] @ dh) / geo["dr"]
+        row = plan.m2 // 2 - (p[:2] @ ch) / geo["dc"]
+        r0, c0 = int(round(row)), int(round(col))
+        win = a[r0 - 2:r0 + 3, c0 - 2:c0 + 3]
+        rr, cc = np.unravel_index(np.argmax(win), win.shape)
+        print(f"target at predicted ({row:.2f}, {col:.2f}): peak ({r0 - 2 + rr}, {c0 - 2 + cc})",
+              flush=True)
+        assert abs(r0 - 2 + rr - row) <= 1.0 and abs(c0 - 2 + cc - col) <= 1.0
+    plan.close()
+
+
+@pytest.mark.parametrize("form", ["bf16", "fp16"])
+def test_fullsize_matrix_core_forms_within_bound(frame, form):
+    import torch
+    spec, sc, geom = frame
+    p32, pf = _plan(frame), _plan(frame, fft=form)
+    i32 = _focus(p32, sc.sig)
+    i16 = _focus(pf, sc.sig)
+    peak = float(i32.abs().max())
+    rms = float(torch.sqrt(torch.mean((i16 - i32).abs().double() ** 2))) / peak
+    print(f"{form}: pn-RMS vs FP32 at 8192^2 = {rms:.2e}", flush=True)
+    assert rms <= 1e-3
+    ka = int(torch.argmax(i32.abs()))
+    kb = int(torch.argmax(i16.abs()))
+    n2 = p32.n2
+    assert abs(ka // n2 - kb // n2) <= 1 and abs(ka % n2 - kb % n2) <= 1
+    p32.close()
+    pf.close()
+
+
+def test_fullsize_detect_equals_abs_of_complex(frame):
+    import torch
+    spec, sc, geom = frame
+    plan = _plan(frame, interp="linear")
+    c = _focus(plan, sc.sig)
+    d = _focus(plan, sc.sig, detect=True)
+    assert torch.allclose(d, c.abs(), rtol=2e-6, atol=1e-6 * float(d.max()))
+    plan.close()
