@@ -346,6 +346,45 @@ __global__ __launch_bounds__(256) void k_range_resample(
 }
 
 // ------------------------------------------------------------------ pass 2 + window + FFT A
+// Two phases per line, each with its own lane assignment:
+//  (A) brackets: thread t owns PER consecutive outputs (the walk from one output
+//      to the next is a step or two); each output's gather recipe -- kind,
+//      sample index, Q15 weight -- is parked in its own slot of `data`;
+//  (B) gathers: the sorted-pulse row is staged in LDS over the dead positions,
+//      and thread t takes outputs t, t + NB, ... so the lanes of a wave read
+//      neighbouring windows (distinct banks) and coalesced tapers; each
+//      output's sinc taps are read as four 16-byte loads.
+// The arithmetic of every output is unchanged: the same bracket, weight and
+// integer gather in the same order.
+__device__ __forceinline__ unsigned gather_code(int kind, int idx, int wq) {
+    return (unsigned)kind << 30 | (unsigned)idx << 15 | (unsigned)wq;   // idx < 2^15, wq < 2^15
+}
+
+// 32-tap sinc of sinc_q15 over an LDS row, the taps of one phase loaded as
+// four 16-byte words (the table row is 64-byte aligned).
+__device__ __forceinline__ short2 sinc_q15_lds(const short2 *row, int idx, const int16_t *taps) {
+    const uint4 *tw = reinterpret_cast<const uint4 *>(taps);
+    uint4 w[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) w[i] = tw[i];
+    int hr = 0, lr = 0, hi = 0, li = 0;
+    const short2 *s = row + idx - 15;
+#pragma unroll
+    for (int t = 0; t < 32; ++t) {
+        const unsigned word = (&w[t >> 3].x)[(t >> 1) & 3];
+        const int c = (t & 1) ? ((int)word >> 16) : (int)(short)(word & 0xffffu);
+        const short2 v = s[t];
+        hr += c * (v.x >> 8);
+        lr += c * (v.x & 255);
+        hi += c * (v.y >> 8);
+        li += c * (v.y & 255);
+    }
+    const int64_t ar = (int64_t)hr * 256 + lr + 16384;
+    const int64_t ai = (int64_t)hi * 256 + li + 16384;
+    return make_short2((short)max(min(ar >> 15, (int64_t)32767), (int64_t)-32768),
+                       (short)max(min(ai >> 15, (int64_t)32767), (int64_t)-32768));
+}
+
 template <int L, bool DO_FFT>
 __global__ __launch_bounds__(Fft<L>::NB) void k_azimuth_fft(
     const short2 *__restrict__ g1t, int ldp, int m, const float *__restrict__ krp,
@@ -355,20 +394,33 @@ __global__ __launch_bounds__(Fft<L>::NB) void k_azimuth_fft(
     int32_t *__restrict__ e_row, int *__restrict__ emax, const short2 *__restrict__ twg) {
     using F = Fft<L>;
     constexpr int G = F::N;
+    constexpr int NB = F::NB;
     __shared__ short2 data[F::DATA];
-    __shared__ float srcv[G];            // sources for the bracket search, then twiddles
+    __shared__ float srcv[G];            // sources for the bracket search, then the row, then twiddles
     __shared__ int red[2 * F::NW];
+    unsigned *codes = reinterpret_cast<unsigned *>(data);
+    short2 *rowl = reinterpret_cast<short2 *>(srcv);
     const int j = blockIdx.x;
     const int t = threadIdx.x;
     const float kr = krp[j];
-    for (int p = t; p < m; p += blockDim.x) srcv[p] = __fmul_rn(kr, tan_s[p]);
+    constexpr int PER = G / NB > 0 ? G / NB : 1;
+    {                                    // positions: all loads in flight before the first product
+        constexpr int CH = PER < 8 ? PER : 8;
+#pragma unroll 1
+        for (int i0 = 0; i0 < PER; i0 += CH) {
+            float ts[CH];
+#pragma unroll
+            for (int i = 0; i < CH; ++i) ts[i] = tan_s[min(t + (i0 + i) * NB, m - 1)];
+#pragma unroll
+            for (int i = 0; i < CH; ++i)
+                if (t + (i0 + i) * NB < m) srcv[t + (i0 + i) * NB] = __fmul_rn(kr, ts[i]);
+        }
+    }
     __syncthreads();
     const bool asc = srcv[m - 1] >= srcv[0];
     auto sa = [&](int u) { return srcv[asc ? u : m - 1 - u]; };
-    const short2 *row = g1t + (size_t)j * ldp;
     const bool use_sinc = flags & PFA_FIXED_AZ_SINC;
-    const int hr = hamr[j];
-    constexpr int PER = G / F::NB > 0 ? G / F::NB : 1;
+    // (A) brackets and weights
     int c = -1;
     for (int u = 0; u < PER; ++u) {
         const int q = t * PER + u;
@@ -395,17 +447,32 @@ __global__ __launch_bounds__(Fft<L>::NB) void k_azimuth_fft(
             }
         }
         const int k = c - 1;
-        short2 v = make_short2(0, 0);
+        unsigned code = 0;                                   // kind 0: zero output
         if (k >= 0 && k <= m - 2) {
             const double x0 = sa(k), x1 = sa(k + 1);
             const double frac = ((double)Q - x0) / (x1 - x0);
             const int idx = asc ? k : m - 2 - k;
             const int wq = q15_weight(asc ? frac : 1.0 - frac);
-            if (use_sinc && idx >= 15 && idx <= m - 17)
-                v = sinc_q15(row, idx, sinc + (wq >> 7) * 32);
-            else
-                v = lerp_q15(row, idx, wq, false);
+            code = gather_code(use_sinc && idx >= 15 && idx <= m - 17 ? 1 : 2, idx, wq);
         }
+        codes[pad32(q)] = code;
+    }
+    __syncthreads();                     // positions dead: the row takes over
+    {
+        const short2 *row = g1t + (size_t)j * ldp;
+        for (int p = t; p < m; p += NB) rowl[p] = row[p];
+    }
+    __syncthreads();
+    // (B) gathers and window
+    const int hr = hamr[j];
+    for (int u = 0; u < PER; ++u) {
+        const int q = t + u * NB;
+        if (q >= G) break;
+        const unsigned code = codes[pad32(q)];
+        const int kind = (int)(code >> 30), idx = (int)((code >> 15) & 0x7fffu), wq = (int)(code & 0x7fffu);
+        short2 v = make_short2(0, 0);
+        if (kind == 1) v = sinc_q15_lds(rowl, idx, sinc + (wq >> 7) * 32);
+        else if (kind == 2) v = lerp_q15(rowl, idx, wq, false);
         if (do_window) {
             const int cw = wrap16((hr * hamc[q]) >> 15);
             v = make_short2((short)wrap16((v.x * cw) >> 15), (short)wrap16((v.y * cw) >> 15));
@@ -414,7 +481,7 @@ __global__ __launch_bounds__(Fft<L>::NB) void k_azimuth_fft(
         else out[(size_t)j * G + q] = v;
     }
     if constexpr (DO_FFT) {
-        __syncthreads();
+        __syncthreads();                 // the row is dead: the twiddles take over
         short2 *tw = reinterpret_cast<short2 *>(srcv);
         load_twiddles<L>(tw, twg);
         __syncthreads();
