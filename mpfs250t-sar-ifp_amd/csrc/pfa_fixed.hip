@@ -56,15 +56,15 @@ struct Fft {
         const int hi = t >> b;
         return lo | (l << b) | (hi << (b + P));
     }
-    static __device__ __forceinline__ int base_of(int s) {     // layout of stage s
+    static constexpr int base_of(int s) {                     // layout of stage s
         const int b = ((s - 1) / P) * P;
         return b < L - P ? b : L - P;
     }
 
     // data: padded natural-order input (read), reused as exchange buffer.
     // On return xr/xi hold the output at idx(t, l, L - P); e = summed shifts.
-    static __device__ void run(short2 *data, const short2 *tw, int *red, int (&xr)[E],
-                               int (&xi)[E], int &e) {
+    static __device__ __forceinline__ void run(short2 *data, const short2 *tw, int *red,
+                                               int (&xr)[E], int (&xi)[E], int &e) {
         const int t = threadIdx.x;
         const bool act = t < NT;
         if (act) {
@@ -78,11 +78,20 @@ struct Fft {
             }
         }
         e = 0;
-        int b = 0;
-#pragma unroll
-        for (int s = 1; s <= L; ++s) {
-            const int nb = base_of(s);
-            if (nb != b) {                      // move the local bit window
+        stages<1>(data, tw, red, xr, xi, e, t, act);
+    }
+
+    // Stage S as straight-line code: its partner bit, half size and bit window
+    // are compile-time constants, so each butterfly names its registers
+    // directly (a run-time stage index would make the register array dynamic).
+    template <int S>
+    static __device__ __forceinline__ void stages(short2 *data, const short2 *tw, int *red,
+                                                  int (&xr)[E], int (&xi)[E], int &e, int t,
+                                                  bool act) {
+        if constexpr (S <= L) {
+            constexpr int b = S == 1 ? 0 : base_of(S - 1);
+            constexpr int nb = base_of(S);
+            if constexpr (nb != b) {                // move the local bit window
                 if (act) {
 #pragma unroll
                     for (int l = 0; l < E; ++l)
@@ -97,21 +106,21 @@ struct Fft {
                         xi[l] = v.y;
                     }
                 }
-                b = nb;
             }
-            const int q = (s - 1) - b;
-            const int h = 1 << (s - 1);
+            constexpr int q = (S - 1) - nb;
+            constexpr int h = 1 << (S - 1);
             int lmax = 0;
             if (act) {
 #pragma unroll
                 for (int l = 0; l < E; ++l) {
                     if (l & (1 << q)) continue;
                     const int l1 = l | (1 << q);
-                    const int j = idx(t, l, b) & (h - 1);
-                    const short2 w = tw[j << (L - s)];
+                    const int j = idx(t, l, nb) & (h - 1);
+                    const short2 w = tw[j << (L - S)];
                     const int br = xr[l1], bi = xi[l1];
-                    const int tr = sat16((w.x * br - w.y * bi) >> 15);
-                    const int ti = sat16((w.x * bi + w.y * br) >> 15);
+                    // |w| < 2^15 and |b| <= 2^15 at a stage's start: 24-bit products are exact
+                    const int tr = sat16((__mul24(w.x, br) - __mul24(w.y, bi)) >> 15);
+                    const int ti = sat16((__mul24(w.x, bi) + __mul24(w.y, br)) >> 15);
                     const int ar = xr[l], ai = xi[l];
                     xr[l] = ar + tr;
                     xi[l] = ai + ti;
@@ -123,11 +132,11 @@ struct Fft {
             }
 #pragma unroll
             for (int off = 32; off >= 1; off >>= 1) lmax = max(lmax, __shfl_xor(lmax, off, 64));
-            if ((t & 63) == 0) red[(s & 1) * NW + (t >> 6)] = lmax;
+            if ((t & 63) == 0) red[(S & 1) * NW + (t >> 6)] = lmax;
             __syncthreads();
-            int mx = red[(s & 1) * NW];
+            int mx = red[(S & 1) * NW];
 #pragma unroll
-            for (int w = 1; w < NW; ++w) mx = max(mx, red[(s & 1) * NW + w]);
+            for (int w = 1; w < NW; ++w) mx = max(mx, red[(S & 1) * NW + w]);
             const int sh = mx <= 32766 ? 0 : (mx <= 65533 ? 1 : 2);
             if (sh && act) {
 #pragma unroll
@@ -137,6 +146,7 @@ struct Fft {
                 }
             }
             e += sh;
+            stages<S + 1>(data, tw, red, xr, xi, e, t, act);
         }
     }
 };
