@@ -260,12 +260,19 @@ __device__ __forceinline__ short2 lerp_q15(const short2 *row, int idx, int wq, b
 // exact integer dot product, round half up, sat16.  Each sample is split into
 // a signed high byte and an unsigned low byte so both partial sums stay in
 // int32 (|c*xh| <= 2^22, |c*xl| <= 2^23 over 32 taps).
-__device__ __forceinline__ short2 sinc_q15(const short2 *row, int idx, const int16_t *taps) {
+// The taps of one phase are loaded as four 16-byte words (a table row is 64
+// bytes, 64-byte aligned).
+__device__ __forceinline__ short2 sinc_q15w(const short2 *row, int idx, const int16_t *taps) {
+    const uint4 *tw = reinterpret_cast<const uint4 *>(taps);
+    uint4 w[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) w[i] = tw[i];
     int hr = 0, lr = 0, hi = 0, li = 0;
     const short2 *s = row + idx - 15;
 #pragma unroll
     for (int t = 0; t < 32; ++t) {
-        const int c = taps[t];
+        const unsigned word = (&w[t >> 3].x)[(t >> 1) & 3];
+        const int c = (t & 1) ? ((int)word >> 16) : (int)(short)(word & 0xffffu);
         const short2 v = s[t];
         hr += c * (v.x >> 8);
         lr += c * (v.x & 255);
@@ -341,7 +348,7 @@ __global__ __launch_bounds__(256) void k_range_resample(
                 const int idx = asc ? k : n - 2 - k;
                 const int wq = q15_weight(asc ? frac : 1.0 - frac);
                 if (use_sinc && idx >= 15 && idx <= n - 17)
-                    v = sinc_q15(row, idx, sinc + (wq >> 7) * 32);
+                    v = sinc_q15w(row, idx, sinc + (wq >> 7) * 32);
                 else
                     v = lerp_q15(row, idx, wq, f32 && !use_sinc);
             }
@@ -368,31 +375,6 @@ __global__ __launch_bounds__(256) void k_range_resample(
 // integer gather in the same order.
 __device__ __forceinline__ unsigned gather_code(int kind, int idx, int wq) {
     return (unsigned)kind << 30 | (unsigned)idx << 15 | (unsigned)wq;   // idx < 2^15, wq < 2^15
-}
-
-// 32-tap sinc of sinc_q15 over an LDS row, the taps of one phase loaded as
-// four 16-byte words (the table row is 64-byte aligned).
-__device__ __forceinline__ short2 sinc_q15_lds(const short2 *row, int idx, const int16_t *taps) {
-    const uint4 *tw = reinterpret_cast<const uint4 *>(taps);
-    uint4 w[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) w[i] = tw[i];
-    int hr = 0, lr = 0, hi = 0, li = 0;
-    const short2 *s = row + idx - 15;
-#pragma unroll
-    for (int t = 0; t < 32; ++t) {
-        const unsigned word = (&w[t >> 3].x)[(t >> 1) & 3];
-        const int c = (t & 1) ? ((int)word >> 16) : (int)(short)(word & 0xffffu);
-        const short2 v = s[t];
-        hr += c * (v.x >> 8);
-        lr += c * (v.x & 255);
-        hi += c * (v.y >> 8);
-        li += c * (v.y & 255);
-    }
-    const int64_t ar = (int64_t)hr * 256 + lr + 16384;
-    const int64_t ai = (int64_t)hi * 256 + li + 16384;
-    return make_short2((short)max(min(ar >> 15, (int64_t)32767), (int64_t)-32768),
-                       (short)max(min(ai >> 15, (int64_t)32767), (int64_t)-32768));
 }
 
 template <int L, bool DO_FFT>
@@ -481,7 +463,7 @@ __global__ __launch_bounds__(Fft<L>::NB) void k_azimuth_fft(
         const unsigned code = codes[pad32(q)];
         const int kind = (int)(code >> 30), idx = (int)((code >> 15) & 0x7fffu), wq = (int)(code & 0x7fffu);
         short2 v = make_short2(0, 0);
-        if (kind == 1) v = sinc_q15_lds(rowl, idx, sinc + (wq >> 7) * 32);
+        if (kind == 1) v = sinc_q15w(rowl, idx, sinc + (wq >> 7) * 32);
         else if (kind == 2) v = lerp_q15(rowl, idx, wq, false);
         if (do_window) {
             const int cw = wrap16((hr * hamc[q]) >> 15);
