@@ -11,6 +11,6 @@ path so that the GPU library can be checked against them:
   SINC32 float variant and float64 forms used as error references.
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
-``bench.py`` may import this package, and only as the checker.  The product
+``bench.py`` may import this package, and only to check results.  The product
 library never calls it and has no CPU fallback through it.
 """

@@ -1,4 +1,4 @@
-"""Build the reference's own C++ FFT model into oracle/_ref/ (checker only).
+"""Build the reference's own C++ FFT model into oracle/_ref/ (test side only).
 
 mpfs/fpga/fft_tb.cpp is the reference's bit-exact C++ restatement of
 fixedpoint.fft1d_bfp (stdlib only).  It is compiled straight from the
