@@ -187,6 +187,18 @@ int pfa_resample_kspace_f32(pfa_plan *plan, const float *sig_c64, const double *
                             float *g2_c64);
 /* focus (form_image_pfa.py:225-240) of a given g2 complex64 [m][n]. */
 int pfa_transform_f32(pfa_plan *plan, const float *g2_c64, void *out, int out_kind);
+/* The O(m n) part of the scene geometry (form_image_pfa.py:187-192,
+ * sar_pipeline.prepare_tables :92-98): kr = (2 freq / c) pr[:, None] in FP64,
+ * each operation rounded as numpy evaluates it, reduced per pulse to
+ * row_max[i] = kr[i].max() and row_min[i] = kr[i].min().  The caller forms
+ * KR's extent from them (kr.max(axis=1).min(), kr.min(axis=1).max(), with
+ * the global extrema as numpy's fallback) -- the same values numpy gives.
+ * Host pointers (synchronous; freq staged like the focus inputs) or device
+ * pointers (_dev, enqueued on the context stream). */
+int pfa_kr_extrema(pfa_ctx *ctx, const double *freq, const double *pr, int m, int n,
+                   double *row_max, double *row_min);
+int pfa_kr_extrema_dev(pfa_ctx *ctx, const double *freq, const double *pr, int m, int n,
+                       double *row_max, double *row_min);
 
 /* ------------------------------------------------------------------ autofocus
  * Gradient autofocus on the float path.  No counterpart in the reference: for

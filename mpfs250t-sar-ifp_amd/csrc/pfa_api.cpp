@@ -748,6 +748,33 @@ extern "C" int pfa_resample_kspace_f32(pfa_plan *p, const float *sig, const doub
     return PFA_OK;
 }
 
+extern "C" int pfa_kr_extrema_dev(pfa_ctx *ctx, const double *freq, const double *pr, int m,
+                                  int n, double *row_max, double *row_min) {
+    if (!ctx) return PFA_ERR_ARG;
+    if (m < 0 || n < 0 || ((size_t)m * n && (!freq || !pr || !row_max || !row_min)))
+        return pfa_fail(ctx, PFA_ERR_ARG, "kr_extrema: bad argument");
+    return launch_kr_extrema(ctx, freq, pr, m, n, row_max, row_min);
+}
+
+extern "C" int pfa_kr_extrema(pfa_ctx *ctx, const double *freq, const double *pr, int m, int n,
+                              double *row_max, double *row_min) {
+    if (!ctx) return PFA_ERR_ARG;
+    if (m < 0 || n < 0 || ((size_t)m * n && (!freq || !pr || !row_max || !row_min)))
+        return pfa_fail(ctx, PFA_ERR_ARG, "kr_extrema: bad argument");
+    if (!m || !n) return PFA_OK;
+    int st = PFA_OK;
+    double *df = (double *)pfa_ws(ctx, "io_in", (size_t)m * n * 8, &st);
+    if (!df) return st;
+    double *dp = (double *)pfa_ws(ctx, "io_pr", (size_t)m * 3 * 8, &st);
+    if (!dp) return st;
+    CK(pfa_io_h2d(ctx, df, freq, (size_t)m * n * 8));
+    CK(pfa_io_h2d(ctx, dp, pr, (size_t)m * 8));
+    CK(launch_kr_extrema(ctx, df, dp, m, n, dp + m, dp + 2 * m));
+    CK(pfa_io_d2h(ctx, row_max, dp + m, (size_t)m * 8));
+    CK(pfa_io_d2h(ctx, row_min, dp + 2 * m, (size_t)m * 8));
+    return PFA_OK;
+}
+
 extern "C" int pfa_transform_f32(pfa_plan *p, const float *g2, void *out, int out_kind) {
     if (!p || p->kind != PLAN_FLOAT) return PFA_ERR_STATE;
     pfa_ctx *ctx = p->ctx;

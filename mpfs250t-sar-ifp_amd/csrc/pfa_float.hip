@@ -121,3 +121,62 @@ int launch_float_fft_b(pfa_plan *p, const void *A, void *out, int out_kind) {
     PFA_HIP(ctx, hipGetLastError());
     return PFA_OK;
 }
+
+// ------------------------------------------------------------------ scene geometry
+// Per-pulse extrema of kr = (2 f / c) pr (the reference's numpy order of
+// operations, each rounded): one workgroup per pulse, loads of a row issued
+// eight at a time from clamped columns.
+namespace {
+constexpr int KX_NB = 256;
+__global__ __launch_bounds__(KX_NB) void k_kr_extrema(const double *__restrict__ freq,
+                                                      const double *__restrict__ pr, int n,
+                                                      double *__restrict__ row_max,
+                                                      double *__restrict__ row_min) {
+    __shared__ double smx[KX_NB / 64], smn[KX_NB / 64];
+    const int i = blockIdx.x, t = threadIdx.x;
+    const double *row = freq + (size_t)i * n;
+    const double a = pr[i];
+    double mx = -INFINITY, mn = INFINITY;
+    for (int j0 = 0; j0 < n; j0 += 8 * KX_NB) {
+        double f[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) f[k] = row[min(j0 + t + k * KX_NB, n - 1)];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const double kr = __dmul_rn(__ddiv_rn(2.0 * f[k], C_LIGHT), a);
+            if (j0 + t + k * KX_NB < n) {
+                mx = fmax(mx, kr);
+                mn = fmin(mn, kr);
+            }
+        }
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        mx = fmax(mx, __shfl_xor(mx, off, 64));
+        mn = fmin(mn, __shfl_xor(mn, off, 64));
+    }
+    if ((t & 63) == 0) {
+        smx[t >> 6] = mx;
+        smn[t >> 6] = mn;
+    }
+    __syncthreads();
+    if (t == 0) {
+        for (int w = 1; w < KX_NB / 64; ++w) {
+            mx = fmax(mx, smx[w]);
+            mn = fmin(mn, smn[w]);
+        }
+        row_max[i] = mx;
+        row_min[i] = mn;
+    }
+}
+}  // namespace
+
+int launch_kr_extrema(pfa_ctx *ctx, const double *freq, const double *pr, int m, int n,
+                      double *row_max, double *row_min) {
+    if (m <= 0 || n <= 0) return PFA_OK;
+    pfa_timer_begin(ctx, PFA_TIMER_OTHER);
+    hipLaunchKernelGGL(k_kr_extrema, dim3(m), dim3(KX_NB), 0, ctx->stream, freq, pr, n, row_max,
+                       row_min);
+    pfa_timer_end(ctx, PFA_TIMER_OTHER);
+    PFA_HIP(ctx, hipGetLastError());
+    return PFA_OK;
+}

@@ -126,6 +126,8 @@ int launch_af_az_adj(pfa_plan *p, const void *g1t, int ldp, const void *rot, con
 int launch_af_grad_reduce(pfa_plan *p, const float *partial, int nparts, const double *stats,
                           float *grad);
 int launch_float_fft_b(pfa_plan *p, const void *A, void *out, int out_kind);
+int launch_kr_extrema(pfa_ctx *ctx, const double *freq, const double *pr, int m, int n,
+                      double *row_max, double *row_min);
 // matrix-core FFT form; fk: 1 BF16, 2 FP16, 3 BF16 with hi/lo DFT matrices
 int launch_float_azimuth_mfma(pfa_plan *p, const void *g1t, int ldp, unsigned *A16, int *expA,
                               int fk);

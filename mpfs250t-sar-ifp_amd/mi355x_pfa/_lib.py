@@ -62,6 +62,8 @@ PROTOS = {
     "pfa_focus_f32_dev": [_vp, _vp, _vp, _vp, _i],
     "pfa_resample_kspace_f32": [_vp, _vp, _vp, _vp],
     "pfa_transform_f32": [_vp, _vp, _vp, _i],
+    "pfa_kr_extrema": [_vp, _vp, _vp, _i, _i, _vp, _vp],
+    "pfa_kr_extrema_dev": [_vp, _vp, _vp, _i, _i, _vp, _vp],
     "pfa_af_checkpoint": [_vp, _vp, _vp],
     "pfa_af_forward": [_vp, _vp, _vp, _vp],
     "pfa_af_backward": [_vp, _vp, _vp, _vp, _vp],

@@ -36,12 +36,38 @@ def to_pow2(n: int) -> int:
     return 1 << int(math.ceil(math.log2(n)))
 
 
-def float_geometry(ax, ay, freq=None, f0=None, df=None, n=None):
-    """form_image_pfa.py:178-193, 205-209, 219-221 on the host.
+def _kr_row_extrema(freq, pr, device=0):
+    """Per-pulse max / min of kr = (2 freq / c) pr[:, None] on the GPU in FP64
+    (numpy's operation order and rounding): the O(m n) part of the geometry.
+    freq: numpy (m, n) float64, or a CUDA float64 tensor."""
+    m, n = freq.shape
+    ctx = context(device)
+    if isinstance(freq, np.ndarray):
+        f = np.ascontiguousarray(freq, np.float64)
+        p = np.ascontiguousarray(pr, np.float64)
+        rmax, rmin = np.empty(m), np.empty(m)
+        ctx.check(_lib.lib().pfa_kr_extrema(ctx.handle, ptr(f), ptr(p), m, n, ptr(rmax),
+                                             ptr(rmin)), "kr_extrema")
+        return rmax, rmin
+    import torch
+    p = torch.as_tensor(np.asarray(pr, np.float64), device=freq.device)
+    rmax = torch.empty(m, dtype=torch.float64, device=freq.device)
+    rmin = torch.empty_like(rmax)
+    torch.cuda.synchronize()
+    ctx.check(_lib.lib().pfa_kr_extrema_dev(ctx.handle, ptr(freq.contiguous()), ptr(p), m, n,
+                                            ptr(rmax), ptr(rmin)), "kr_extrema")
+    ctx.synchronize()
+    return rmax.cpu().numpy(), rmin.cpu().numpy()
 
-    Either the full freq (m, n) array, or compact per-pulse f0/df with
-    freq[i, k] = f0[i] + k * df[i] (rows then monotone, so each row's extrema
-    of kr are its end points)."""
+
+def float_geometry(ax, ay, freq=None, f0=None, df=None, n=None, device=0):
+    """form_image_pfa.py:178-193, 205-209, 219-221.
+
+    The O(m + n) tables are formed on the host with the reference's numpy
+    operations.  With the full freq (m, n) array (numpy or a CUDA tensor) the
+    O(m n) per-pulse extrema of kr are reduced on the GPU in FP64 (same values);
+    with compact per-pulse f0/df (freq[i, k] = f0[i] + k * df[i], rows monotone)
+    each row's extrema are its end points."""
     ax = np.asarray(ax, np.float64)
     ay = np.asarray(ay, np.float64)
     dx, dy = ax.mean(), ay.mean()
@@ -52,11 +78,11 @@ def float_geometry(ax, ay, freq=None, f0=None, df=None, n=None):
     pc = ax * cx + ay * cy
     tan_phi = pc / pr
     if freq is not None:
-        freq = np.asarray(freq, np.float64)
+        if isinstance(freq, np.ndarray) or not hasattr(freq, "is_cuda"):
+            freq = np.asarray(freq, np.float64)
         n = freq.shape[1]
-        kr = (2.0 * freq / C_LIGHT) * pr[:, None]
-        row_max, row_min = kr.max(axis=1), kr.min(axis=1)
-        gmin, gmax = kr.min(), kr.max()
+        row_max, row_min = _kr_row_extrema(freq, pr, device)
+        gmin, gmax = row_min.min(), row_max.max()
     else:
         f0 = np.asarray(f0, np.float64)
         df = np.asarray(df, np.float64)

@@ -129,6 +129,19 @@ def test_quicklook_refuses_other_sizes(gp):
         gp.quicklook(np.zeros((300, 256), np.complex64), -1)
 
 
+@pytest.mark.parametrize("desc", [False, True])
+def test_geometry_extrema_on_gpu_equal_numpy(gp, desc):
+    """The O(m n) part of the geometry (per-pulse extrema of kr) reduced on the
+    GPU gives the oracle's KR / KC grids bit for bit, from a host array and from
+    a device tensor."""
+    import torch
+    sc, freq, ax, ay = _scene(700, 900, seed=21, desc=desc)
+    ref = rfl.geometry(freq, ax, ay)
+    for f in (freq, torch.from_numpy(np.ascontiguousarray(freq)).cuda()):
+        g = gp.float_geometry(ax, ay, freq=f)
+        assert np.array_equal(g["KR"], ref["KR"]) and np.array_equal(g["KC"], ref["KC"])
+
+
 def test_host_buffer_path_equals_device_path(gp):
     """The host-buffer entry points stage large copies through a ring of pinned
     buffers on a copy stream: 18.5 MB of signal and of frequencies in (two chunks,
