@@ -409,6 +409,20 @@ def main():
         plan.focus(sig_host, detect=True)
     res["end_to_end_host_buffers_ms"] = round((time.perf_counter() - t0) / 3 * 1e3, 2)
     del sig_host
+    # scene geometry from the full freq (m, n) array: the O(m n) kr extrema on
+    # the GPU, from a host array (staged) and from a device tensor
+    from mi355x_pfa.pfa import float_geometry
+    freq_host = sc.freq()
+    freq_dev = torch.from_numpy(freq_host).to("cuda")
+    g = {}
+    for name, f in (("host_array", freq_host), ("device_tensor", freq_dev)):
+        float_geometry(ax, ay, freq=f)
+        t0 = time.perf_counter()
+        for _ in range(3):
+            float_geometry(ax, ay, freq=f)
+        g[name] = round((time.perf_counter() - t0) / 3 * 1e3, 2)
+    res["geometry_full_freq_ms"] = g
+    del freq_host, freq_dev
     del out
     plan.close()
     if a.copy_calibration:
