@@ -76,6 +76,28 @@ _lib = None
 _lock = threading.Lock()
 
 
+def _share_torch_runtime():
+    """PyTorch-ROCm wheels carry their own HIP runtime (libamdhip64.so,
+    libhsa-runtime64.so under torch/lib).  If the library bound the system
+    runtime first, a later ``import torch`` in the same process would load the
+    second copy, whose device initialisation then fails ("No HIP GPUs are
+    available").  When torch is installed, its runtime is loaded globally
+    before the library, so the library's sonames resolve to it and the process
+    holds one runtime whichever is imported first."""
+    import importlib.util
+    try:
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        return
+    if spec is None or not spec.submodule_search_locations:
+        return
+    d = Path(list(spec.submodule_search_locations)[0]) / "lib"
+    for name in ("libhsa-runtime64.so", "libamdhip64.so"):
+        p = d / name
+        if p.exists():
+            C.CDLL(str(p), mode=C.RTLD_GLOBAL)
+
+
 def lib():
     """Load the shared library once; raise loudly if it is missing."""
     global _lib
@@ -83,6 +105,7 @@ def lib():
         if _lib is None:
             if not LIB_PATH.exists():
                 raise PfaError(-4, f"{LIB_PATH} is not built (run mpfs250t-sar-ifp_amd/build.py)")
+            _share_torch_runtime()
             L = C.CDLL(str(LIB_PATH))
             for name, args in PROTOS.items():
                 f = getattr(L, name)
