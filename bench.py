@@ -423,6 +423,15 @@ def main():
         g[name] = round((time.perf_counter() - t0) / 3 * 1e3, 2)
     res["geometry_full_freq_ms"] = g
     del freq_host, freq_dev
+    # the pipeline's table preparation (sar_pipeline.prepare_tables) from the
+    # scene's PVP arrays, with no (m, n) host array
+    from mi355x_pfa.pipeline import prepare_tables
+    reader = synth.SceneReader(sc)
+    prepare_tables(reader, reader.cphd_meta)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        prepare_tables(reader, reader.cphd_meta)
+    res["prepare_tables_ms"] = round((time.perf_counter() - t0) / 3 * 1e3, 2)
     del out
     plan.close()
     if a.copy_calibration:
@@ -458,6 +467,11 @@ def main():
     if not a.no_cpu:
         print("[bench] cpu baseline (numpy oracle, bounded sample)", file=sys.stderr, flush=True)
         res["cpu_baseline"] = cpu_baseline(spec, a.interp, geom)
+        # the reference's numpy table preparation on the same scene, O(m n)
+        from oracle import ref_float as rfl
+        t0 = time.perf_counter()
+        rfl.prepare_tables(reader, reader.cphd_meta)
+        res["cpu_baseline"]["prepare_tables_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
     print(json.dumps(res))
 
 

@@ -60,18 +60,21 @@ def _kr_row_extrema(freq, pr, device=0):
     return rmax.cpu().numpy(), rmin.cpu().numpy()
 
 
-def float_geometry(ax, ay, freq=None, f0=None, df=None, n=None, device=0):
+def float_geometry(ax, ay, freq=None, f0=None, df=None, n=None, device=0, k0=0,
+                   _hypot=math.hypot):
     """form_image_pfa.py:178-193, 205-209, 219-221.
 
     The O(m + n) tables are formed on the host with the reference's numpy
     operations.  With the full freq (m, n) array (numpy or a CUDA tensor) the
     O(m n) per-pulse extrema of kr are reduced on the GPU in FP64 (same values);
-    with compact per-pulse f0/df (freq[i, k] = f0[i] + k * df[i], rows monotone)
-    each row's extrema are its end points."""
+    with compact per-pulse f0/df (freq[i, k] = f0[i] + (k0 + k) * df[i], rows
+    monotone) each row's extrema are its end points.  ``_hypot`` is the norm of
+    the mean look direction: math.hypot in form_image_pfa.py:181, np.hypot in
+    sar_pipeline.py:95."""
     ax = np.asarray(ax, np.float64)
     ay = np.asarray(ay, np.float64)
     dx, dy = ax.mean(), ay.mean()
-    dn = math.hypot(dx, dy)
+    dn = _hypot(dx, dy)
     dx, dy = dx / dn, dy / dn
     cx, cy = -dy, dx
     pr = ax * dx + ay * dy
@@ -86,7 +89,8 @@ def float_geometry(ax, ay, freq=None, f0=None, df=None, n=None, device=0):
     else:
         f0 = np.asarray(f0, np.float64)
         df = np.asarray(df, np.float64)
-        ends = np.stack([f0, f0 + (n - 1) * df], axis=1)
+        kk = np.array([k0, k0 + n - 1])
+        ends = f0[:, None] + kk[None, :] * df[:, None]
         kr = (2.0 * ends / C_LIGHT) * pr[:, None]
         row_max, row_min = kr.max(axis=1), kr.min(axis=1)
         gmin, gmax = kr.min(), kr.max()
@@ -266,9 +270,25 @@ class GpuBackend:
         self.fft = fft
 
     def focus(self, signal, tables):
-        plan = _plan_for(signal, tables["freq"], tables["ax"], tables["ay"], tables["sgn"],
-                         True, self.interp, self.device, self.fft)
-        out = plan.focus(signal, tables["freq"], detect=True)
+        """tables from sar_pipeline.prepare_tables (freq an (m, n) array) or
+        from pipeline.prepare_tables (freq a LinearFreq: only the per-pulse
+        sc0 / step reach the GPU when no samples were cropped)."""
+        from .pipeline import LinearFreq
+        fr = tables["freq"]
+        if isinstance(fr, LinearFreq) and fr.k0 == 0:
+            m, n = np.shape(signal)
+            if fr.shape != (m, n):
+                raise PfaError(-1, f"freq shape {fr.shape} != signal shape {(m, n)}")
+            geom = float_geometry(tables["ax"], tables["ay"], f0=fr.sc0, df=fr.step, n=n)
+            plan = FloatPlan(m, n, tables["sgn"], geom, f0=fr.sc0, df=fr.step, window=True,
+                             interp=self.interp, device=self.device, fft=self.fft)
+            out = plan.focus(signal, None, detect=True)
+        else:
+            if isinstance(fr, LinearFreq):            # cropped: freq starts at sample k0
+                fr = np.asarray(fr)
+            plan = _plan_for(signal, fr, tables["ax"], tables["ay"], tables["sgn"],
+                             True, self.interp, self.device, self.fft)
+            out = plan.focus(signal, fr, detect=True)
         plan.close()
         return out
 

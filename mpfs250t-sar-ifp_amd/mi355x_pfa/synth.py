@@ -173,3 +173,45 @@ def make_scene(spec: SceneSpec, device=None, chunk_targets=32) -> Scene:
         scene.sig = torch.complex(acc_r.float(), acc_i.float())
         del acc_r, acc_i
     return scene
+
+
+class _NS:
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+
+class SceneReader:
+    """A scene presented as the CPHD reader the reference's table preparation
+    reads (sar_pipeline.py:61-75, form_image_pfa.py:326-343, 346-348):
+    ``get_data_size_as_tuple()``, ``read_pvp_variable(name, 0)`` for TxPos,
+    RcvPos, SRPPos, SC0 and SCSS, ``read_chip(dim1, dim2, index=0)`` with
+    (start, stop, step) ranges, ``cphd_meta`` (Global.SGN, Global.DomainType,
+    SceneCoordinates.ReferenceSurface.Planar.uIAX / uIAY) and ``close()``."""
+
+    def __init__(self, scene: Scene):
+        self.scene = scene
+        vec = lambda v: _NS(X=float(v[0]), Y=float(v[1]), Z=float(v[2]))  # noqa: E731
+        planar = _NS(uIAX=vec(scene.uiax), uIAY=vec(scene.uiay))
+        self.cphd_meta = _NS(Global=_NS(SGN=scene.sgn, DomainType="FX"),
+                             SceneCoordinates=_NS(ReferenceSurface=_NS(Planar=planar)))
+        self._pvp = {"TxPos": scene.tx, "RcvPos": scene.rcv, "SRPPos": scene.srp,
+                     "SC0": scene.sc0, "SCSS": scene.scss}
+
+    def get_data_size_as_tuple(self):
+        return ((self.scene.spec.pulses, self.scene.spec.samples),)
+
+    def read_pvp_variable(self, name, index=0):
+        if index != 0:
+            raise IndexError("one channel")
+        return np.array(self._pvp[name], copy=True)
+
+    def read_chip(self, dim1_range, dim2_range, index=0):
+        if index != 0:
+            raise IndexError("one channel")
+        sig = self.scene.sig
+        if not isinstance(sig, np.ndarray):
+            sig = sig.cpu().numpy()
+        return np.ascontiguousarray(sig[slice(*dim1_range), slice(*dim2_range)])
+
+    def close(self):
+        pass

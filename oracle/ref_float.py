@@ -8,6 +8,8 @@ TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
 * ``focus``            -- form_image_pfa.py:225-240 with transform2d :144-147
 * ``pfa``              -- form_image_pfa.py:243-246
 * ``quicklook``        -- form_image_pfa.py:154-162 (hamming2d + transform2d)
+* ``prepare_tables``   -- sar_pipeline.py:36-115 (PVP geometry -> focus inputs,
+                          KR / KC grids, taper, pixel geometry; O(m n) in numpy)
 
 ``store`` selects the intermediate precision: complex64 reproduces the
 reference; complex128 gives the float64 oracle that the reduced-precision
@@ -167,3 +169,56 @@ def quicklook(sig, sgn, window=True):
 def detect(img):
     """accel.py:44: |img| as float32."""
     return np.abs(img).astype(np.float32)
+
+
+def prepare_tables(reader, meta, deci_pulse=1, deci_sample=1, crop_sample=0):
+    """sar_pipeline.py:36-115: read the PVP geometry of a CPHD reader and form
+    the focus inputs (ax, ay, the (m, n) freq array, sgn), the resample grids
+    KR / KC, tan_phi, the float32 outer Hamming taper and the pixel geometry.
+    Every array is built the way numpy builds it there, including the O(m n)
+    kr array whose row extrema set KR's extent, and ``np.hypot`` for the mean
+    look direction's norm (form_image_pfa.py:181 uses ``math.hypot``)."""
+    mu, nu = max(1, deci_pulse), max(1, deci_sample)
+    n_vec, n_samp = reader.get_data_size_as_tuple()[0]
+    sgn = int(meta.Global.SGN)
+    tx = reader.read_pvp_variable("TxPos", 0)[::mu]
+    rcv = reader.read_pvp_variable("RcvPos", 0)[::mu]
+    srp = reader.read_pvp_variable("SRPPos", 0)[::mu]
+    sc0 = reader.read_pvp_variable("SC0", 0)[::mu]
+    scss = reader.read_pvp_variable("SCSS", 0)[::mu]
+    apc = 0.5 * (tx + rcv)
+    planar = meta.SceneCoordinates.ReferenceSurface.Planar
+    uiax = np.array([planar.uIAX.X, planar.uIAX.Y, planar.uIAX.Z])
+    uiay = np.array([planar.uIAY.X, planar.uIAY.Y, planar.uIAY.Z])
+    los = srp - apc
+    u = los / np.linalg.norm(los, axis=1, keepdims=True)
+    ax, ay = u @ uiax, u @ uiay
+    m, n = len(range(0, n_vec, mu)), len(range(0, n_samp, nu))
+    d0 = 0
+    if crop_sample and crop_sample < n:
+        d0 = (n - crop_sample) // 2
+        n = crop_sample
+    k = d0 + np.arange(n)
+    freq = sc0[:, None] + k[None, :] * (scss[:, None] * nu)
+    kmag = 2.0 * freq / C_LIGHT
+    dx, dy = ax.mean(), ay.mean()
+    dn = np.hypot(dx, dy)
+    dx, dy = dx / dn, dy / dn
+    cx, cy = -dy, dx
+    pr = ax * dx + ay * dy
+    pc = ax * cx + ay * cy
+    kr = kmag * pr[:, None]
+    tan_phi = pc / pr
+    lo, hi = kr.max(axis=1).min(), kr.min(axis=1).max()
+    if lo > hi:
+        lo, hi = kr.min(), kr.max()
+    KR = np.linspace(lo, hi, n)
+    kc_max = np.abs(KR[:, None] * tan_phi[None, :]).max()
+    KC = np.linspace(-kc_max, kc_max, m)
+    window = np.outer(np.hamming(m), np.hamming(n)).astype(np.float32)
+    geo = {"dc": 1.0 / (KC[-1] - KC[0]), "dr": 1.0 / (KR[-1] - KR[0]),
+           "dhat": (dx, dy), "chat": (cx, cy)}
+    return {"ax": ax, "ay": ay, "freq": freq, "sgn": sgn, "KR": KR, "KC": KC,
+            "tan_phi": tan_phi, "window": window, "geo": geo, "center_ecef": srp[0],
+            "uiax": uiax, "uiay": uiay, "dims": (m, n), "deci": (mu, nu), "n_vec": n_vec,
+            "n_samp": n_samp, "samp0": d0 * nu}
