@@ -200,6 +200,42 @@ int pfa_kr_extrema(pfa_ctx *ctx, const double *freq, const double *pr, int m, in
 int pfa_kr_extrema_dev(pfa_ctx *ctx, const double *freq, const double *pr, int m, int n,
                        double *row_max, double *row_min);
 
+/* ------------------------------------------------------------------ display
+ * The passes after detection that turn an image into a product.  Host
+ * pointers (synchronous, staged copies) or device pointers (_dev, enqueued on
+ * the context stream unless stated).
+ *
+ * Normal-score stretch of a uint16 magnitude image (sar_display.normal_score,
+ * mpfs/host/sar_display.py:24-60):
+ *   pfa_hist_u16    hist[65536] = np.bincount(img, minlength=65536) as uint32
+ *                   (the counts behind lo, hi and the midpoint CDF, :30-33)
+ *   pfa_lut_u16_u8  out[i] = lut[img[i] - lo], lut[nb] uint8, 1 <= nb <= 65536
+ *                   (:59-60); values outside [lo, lo + nb) take the nearest
+ *                   table end, where numpy would wrap or raise
+ * The LUT itself (CDF, inverse normal CDF, clip) is O(nb) host work. */
+int pfa_hist_u16(pfa_ctx *ctx, const uint16_t *img, int64_t count, uint32_t *hist);
+int pfa_hist_u16_dev(pfa_ctx *ctx, const uint16_t *img, int64_t count, uint32_t *hist);
+int pfa_lut_u16_u8(pfa_ctx *ctx, const uint16_t *img, int64_t count, const uint8_t *lut,
+                   int lo, int nb, uint8_t *out);
+int pfa_lut_u16_u8_dev(pfa_ctx *ctx, const uint16_t *img, int64_t count, const uint8_t *lut,
+                       int lo, int nb, uint8_t *out);
+/* Order statistics for np.percentile (the 99.7 % clip level of the uint8
+ * GeoTIFF product, src/form_image_pfa.py:288): v0 and v1 are the k0-th and
+ * k1-th smallest of x[count] (0-based ranks, NaN ordered last as np.partition
+ * does; a -0.0 / +0.0 tie may return either zero), nan_count the number of
+ * NaNs (may be NULL).  v0, v1 and nan_count are host pointers in both forms;
+ * the _dev form reads x from the device and synchronises the stream. */
+int pfa_select_f32(pfa_ctx *ctx, const float *x, int64_t count, int64_t k0, int64_t k1,
+                   float *v0, float *v1, int64_t *nan_count);
+int pfa_select_f32_dev(pfa_ctx *ctx, const float *x, int64_t count, int64_t k0, int64_t k1,
+                       float *v0, float *v1, int64_t *nan_count);
+/* uint8 product (src/form_image_pfa.py:289): out[i] = uint8(clip(x[i] / d *
+ * 255, 0, 255)) with d = hi + 1e-12 formed by the caller in float32; each
+ * operation rounded in float32 as numpy rounds it, truncation to uint8, NaN
+ * -> 0. */
+int pfa_stretch_u8(pfa_ctx *ctx, const float *x, int64_t count, float d, uint8_t *out);
+int pfa_stretch_u8_dev(pfa_ctx *ctx, const float *x, int64_t count, float d, uint8_t *out);
+
 /* ------------------------------------------------------------------ autofocus
  * Gradient autofocus on the float path.  No counterpart in the reference: for
  * callers that focus with per-pulse phase corrections phi[m] (original pulse

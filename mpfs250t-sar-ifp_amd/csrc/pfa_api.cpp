@@ -212,12 +212,6 @@ static T *upload(pfa_plan *p, const T *h, size_t count, int *st) {
     return (T *)d;
 }
 
-#define CK(x)                     \
-    do {                          \
-        int s_ = (x);             \
-        if (s_ != PFA_OK) return s_; \
-    } while (0)
-
 // ------------------------------------------------------------------ fixed: stage functions
 extern "C" int pfa_quantize_iq16_dev(pfa_ctx *ctx, const float *sig, int64_t count, int16_t *out,
                                      int *exp_out) {

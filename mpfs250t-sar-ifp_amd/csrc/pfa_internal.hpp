@@ -80,6 +80,12 @@ int pfa_io_h2d(pfa_ctx *ctx, void *dst, const void *src, size_t bytes);
 int pfa_io_d2h(pfa_ctx *ctx, void *dst, const void *src, size_t bytes);
 void pfa_hostio_destroy(pfa_ctx *ctx);
 
+#define CK(x)                        \
+    do {                             \
+        int s_ = (x);                \
+        if (s_ != PFA_OK) return s_; \
+    } while (0)
+
 #define PFA_HIP(ctx, call)                                        \
     do {                                                          \
         hipError_t e_ = (call);                                   \

@@ -64,6 +64,16 @@ PROTOS = {
     "pfa_transform_f32": [_vp, _vp, _vp, _i],
     "pfa_kr_extrema": [_vp, _vp, _vp, _i, _i, _vp, _vp],
     "pfa_kr_extrema_dev": [_vp, _vp, _vp, _i, _i, _vp, _vp],
+    "pfa_hist_u16": [_vp, _vp, _i64, _vp],
+    "pfa_hist_u16_dev": [_vp, _vp, _i64, _vp],
+    "pfa_lut_u16_u8": [_vp, _vp, _i64, _vp, _i, _i, _vp],
+    "pfa_lut_u16_u8_dev": [_vp, _vp, _i64, _vp, _i, _i, _vp],
+    "pfa_select_f32": [_vp, _vp, _i64, _i64, _i64, C.POINTER(C.c_float),
+                       C.POINTER(C.c_float), C.POINTER(_i64)],
+    "pfa_select_f32_dev": [_vp, _vp, _i64, _i64, _i64, C.POINTER(C.c_float),
+                           C.POINTER(C.c_float), C.POINTER(_i64)],
+    "pfa_stretch_u8": [_vp, _vp, _i64, C.c_float, _vp],
+    "pfa_stretch_u8_dev": [_vp, _vp, _i64, C.c_float, _vp],
     "pfa_af_checkpoint": [_vp, _vp, _vp],
     "pfa_af_forward": [_vp, _vp, _vp, _vp],
     "pfa_af_backward": [_vp, _vp, _vp, _vp, _vp],
