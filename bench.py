@@ -88,7 +88,7 @@ def time_float(plan, sig, out, steps, warmup, torch):
     return wall, kern
 
 
-def time_fixed(sc, ax, ay, geom, steps, warmup, torch):
+def time_fixed(sc, ax, ay, geom, steps, warmup, torch, product=None):
     from mi355x_pfa import _lib
     from mi355x_pfa.emulator import FixedPlan
     from mi355x_pfa.tables import fixed_tables
@@ -123,8 +123,48 @@ def time_fixed(sc, ax, ay, geom, steps, warmup, torch):
     ctx.timer_enable(False)
     ctx.timer_reset()
     plan.close()
+    if product is not None:                       # normal score of the uint16 [range][cross] image
+        product["normal_score"] = time_product("normal_score", out.view(torch.uint16), torch,
+                                               product.pop("_with_cpu"))
     del sig_iq, out
     return wall, kern
+
+
+def time_product(kind, img, torch, with_cpu, reps=5):
+    """A product pass on a device-resident detected frame: "geotiff_u8" (the
+    99.7-percentile uint8 data of save_detected_geotiff) on a float32 image, or
+    "normal_score" (sar_display.normal_score) on a uint16 image.  Wall time per
+    call (host syncs, the O(bins) host steps and the small copies included) and
+    the GPU kernels' event time; with_cpu also times the numpy oracle on the
+    same frame and checks the outputs are equal."""
+    from mi355x_pfa import _lib, display
+    from mi355x_pfa._lib import context
+    ctx = context(0)
+    fn = (lambda: display.stretch_u8(img)[0]) if kind == "geotiff_u8" else \
+        (lambda: display.normal_score(img))
+    dev_out = fn()
+    torch.cuda.synchronize()
+    ctx.timer_reset()
+    ctx.timer_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / reps * 1e3
+    ms, _ = ctx.timer_read(_lib.TIMER_OTHER)
+    ctx.timer_enable(False)
+    ctx.timer_reset()
+    n = img.numel()
+    res = {"ms": round(wall, 3), "kernels_ms": round(ms / reps, 4), "pixels": n,
+           "dtype_in": str(img.dtype).replace("torch.", "")}
+    if with_cpu:
+        from oracle import ref_display as rd
+        host = img.cpu().numpy()
+        t0 = time.perf_counter()
+        ref = rd.geotiff_u8(host)[0] if kind == "geotiff_u8" else rd.normal_score(host)
+        res["numpy_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        res["equal_to_numpy"] = bool(np.array_equal(dev_out.cpu().numpy(), ref))
+    return res
 
 
 def _fp64_reference(spec, sc, geom, interp, torch):
@@ -400,6 +440,8 @@ def main():
         "kernels_gbps": {k: round(bytes_k[k] / (kern[k] * 1e-3) / 1e9, 1) for k in kern},
         "fallback_outputs_per_frame": [round(c / (a.steps + a.warmup)) for c in fb],
     }
+    print("[bench] product passes", file=sys.stderr, flush=True)
+    res["products"] = {"geotiff_u8": time_product("geotiff_u8", out, torch, not a.no_cpu)}
     # end to end through the host-buffer entry point (pageable numpy in and
     # out, as the reference's ctypes caller would pass them): never `value`
     sig_host = sc.sig.cpu().numpy()
@@ -460,7 +502,8 @@ def main():
         res["autofocus"] = autofocus_c5(a.interp, a.autofocus_steps, torch)
     if not a.no_fixed and a.config == "C3":
         print("[bench] fixed-point frames", file=sys.stderr, flush=True)
-        fwall, fk = time_fixed(sc, ax, ay, geom, max(3, a.steps // 2), 2, torch)
+        res["products"]["_with_cpu"] = not a.no_cpu
+        fwall, fk = time_fixed(sc, ax, ay, geom, max(3, a.steps // 2), 2, torch, res["products"])
         res["fixed"] = {"ms_per_frame": round(fwall, 3),
                         "kernels_ms": {k: round(v, 4) for k, v in fk.items()},
                         "speedup_vs_fpga_silicon": round(FPGA_FRAME_MS / fwall, 1)}
