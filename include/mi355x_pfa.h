@@ -156,7 +156,10 @@ int pfa_form_image_fixed_dev(pfa_plan *plan, const int16_t *sig_iq, uint16_t *ou
  *   ham_c[m], ham_r[n]  np.hamming(m), np.hamming(n) (NULL: no window)
  *   sinc_tab       256x32 Q15 table, needed with PFA_FLOAT_SINC32
  *   sgn            <0: fft2 (SGN=-1), >0: ifft2 (transform2d :144-147)
- * The image is the zero-padded (to_pow2) centred transform, [cross][range]. */
+ * The image is the zero-padded (to_pow2) centred transform, [cross][range],
+ * m2 = max(64, pow2(m)) <= 8192 (the cross line is transformed in LDS with its
+ * resample) and n2 = max(64, pow2(n)) <= 16384 (the range transform takes one
+ * line per workgroup); larger scenes return PFA_ERR_DIMS. */
 #define PFA_FLOAT_SINC32 0x1        /* 32-tap polyphase gather both passes     */
 /* Precision form of pfa_focus_f32[_dev] (no reference counterpart; the FP32
  * form above is the reference's arithmetic): both FFT passes as dense

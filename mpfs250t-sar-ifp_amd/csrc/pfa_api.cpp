@@ -507,7 +507,10 @@ extern "C" int pfa_plan_float(pfa_ctx *ctx, pfa_plan **out, int m, int n, int sg
     if (sgn == 0) return pfa_fail(ctx, PFA_ERR_ARG, "plan_float: sgn must be -1 or +1");
     if (m < 2 || n < 2) return pfa_fail(ctx, PFA_ERR_DIMS, "plan_float: need m, n >= 2");
     const int m2 = std::max(64, pow2_ceil(m)), n2 = std::max(64, pow2_ceil(n));
-    if (m2 > 8192 || n2 > 8192) return pfa_fail(ctx, PFA_ERR_DIMS, "plan_float: padded grid above 8192");
+    // the cross line (m2) is transformed in LDS with its resample; the range
+    // transform (n2) is one line per workgroup and reaches 16384 (FP32 form)
+    if (m2 > 8192) return pfa_fail(ctx, PFA_ERR_DIMS, "plan_float: padded pulse (cross) size above 8192");
+    if (n2 > 16384) return pfa_fail(ctx, PFA_ERR_DIMS, "plan_float: padded sample (range) size above 16384");
     if (pow2_ceil(m) != m2 || pow2_ceil(n) != n2)
         return pfa_fail(ctx, PFA_ERR_DIMS, "plan_float: m and n must be at least 33 (padded size >= 64)");
     if (flags & (PFA_FLOAT_FFT_BF16 | PFA_FLOAT_FFT_FP16)) {
@@ -631,6 +634,7 @@ static int af_check(pfa_plan *p) {
     if (p->flags & (PFA_FLOAT_FFT_BF16 | PFA_FLOAT_FFT_FP16))
         return pfa_fail(p->ctx, PFA_ERR_ARG, "autofocus: needs the FP32 FFT form");
     if (p->m2 > 4096) return pfa_fail(p->ctx, PFA_ERR_DIMS, "autofocus: padded cross size above 4096");
+    if (p->n2 > 8192) return pfa_fail(p->ctx, PFA_ERR_DIMS, "autofocus: padded range size above 8192");
     return PFA_OK;
 }
 

@@ -34,6 +34,12 @@
         MACRO(6) MACRO(7) MACRO(8) MACRO(9) MACRO(10) MACRO(11) MACRO(12) MACRO(13) \
         default: return pfa_fail(ctx, PFA_ERR_DIMS, "padded size must be a power of two in 64..8192"); \
     }
+// the range transform (FFT-B) also takes 16384-point lines
+#define PFA_F32_SWITCH_B(L, MACRO) \
+    switch (L) {                   \
+        MACRO(6) MACRO(7) MACRO(8) MACRO(9) MACRO(10) MACRO(11) MACRO(12) MACRO(13) MACRO(14) \
+        default: return pfa_fail(ctx, PFA_ERR_DIMS, "padded range size must be a power of two in 64..16384"); \
+    }
 
 int launch_float_range(pfa_plan *p, const void *sig, const double *freq, void *g1t, int ldp) {
     pfa_ctx *ctx = p->ctx;
@@ -114,7 +120,7 @@ int launch_float_fft_b(pfa_plan *p, const void *A, void *out, int out_kind) {
         else if (out_kind) hipLaunchKernelGGL((k_fft_b2_f32<LL, false, 1>), dim3(p->m2 / 2), dim3(fb2_nb<LL>()), 0, ctx->stream, ARGS); \
         else hipLaunchKernelGGL((k_fft_b2_f32<LL, false, 0>), dim3(p->m2 / 2), dim3(fb2_nb<LL>()), 0, ctx->stream, ARGS); \
         break;
-    PFA_F32_SWITCH(L, LAUNCH)
+    PFA_F32_SWITCH_B(L, LAUNCH)
 #undef LAUNCH
 #undef ARGS
     pfa_timer_end(ctx, PFA_TIMER_FFT_B);

@@ -233,13 +233,15 @@ def focus(g2, geo, sgn, device=0):
 def quicklook(sig, sgn, window=True, device=0):
     """form_image_pfa.quicklook -> complex64 (m, n): window + one centred 2-D
     FFT of the raw history (the polar grid taken as Cartesian), no padding.
-    Both sizes must be powers of two (256 ... 8192); the reference's numpy FFT
-    takes any size, and other sizes are refused here."""
+    Both sizes must be powers of two (pulses 64 ... 8192, samples 64 ...
+    16384); the reference's numpy FFT takes any size, and other sizes are
+    refused here."""
     sig = np.ascontiguousarray(sig, np.complex64)
     m, n = sig.shape
-    for s in (m, n):
-        if s < 2 or s & (s - 1) or s > 8192:
-            raise PfaError(-1, f"quicklook sizes must be powers of two up to 8192, got {m}x{n}")
+    for s, top in ((m, 8192), (n, 16384)):
+        if s < 2 or s & (s - 1) or s > top:
+            raise PfaError(-1, f"quicklook sizes must be powers of two up to 8192 pulses x "
+                               f"16384 samples, got {m}x{n}")
     dummy = {"pr": np.ones(m), "order": np.arange(m, dtype=np.int32), "tan_s": np.zeros(m),
              "KR": np.ones(n), "KC": np.zeros(m), "geo": {}}
     plan = FloatPlan(m, n, sgn, dummy, window=window, device=device)

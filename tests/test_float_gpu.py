@@ -77,7 +77,8 @@ def _scene(m, n, seed, targets=16, half=None, desc=False):
 
 
 @pytest.mark.parametrize("m,n,sgn,desc", [(1024, 1024, -1, False), (700, 1100, 1, False),
-                                          (300, 257, -1, True)])
+                                          (300, 257, -1, True), (600, 9000, -1, False),
+                                          (300, 8300, 1, True)])
 def test_linear_pfa_matches_oracle(gp, m, n, sgn, desc):
     sc, freq, ax, ay = _scene(m, n, seed=m + n, desc=desc)
     print(f"linear pfa {m}x{n} sgn={sgn} desc={desc}", flush=True)
@@ -87,7 +88,7 @@ def test_linear_pfa_matches_oracle(gp, m, n, sgn, desc):
     assert top_peaks(img, 8) == top_peaks(ref, 8)
 
 
-@pytest.mark.parametrize("m,n", [(256, 300), (1024, 1024)])
+@pytest.mark.parametrize("m,n", [(256, 300), (1024, 1024), (256, 8200)])
 def test_sinc32_pfa_matches_fp64_oracle(gp, m, n):
     sc, freq, ax, ay = _scene(m, n, seed=3 * m + n)
     print(f"sinc32 pfa {m}x{n}", flush=True)
@@ -112,7 +113,7 @@ def test_compact_frequencies_equal_full_array(gp):
 
 
 @pytest.mark.parametrize("m,n,sgn,window", [(256, 512, -1, True), (1024, 1024, 1, True),
-                                            (512, 256, -1, False)])
+                                            (512, 256, -1, False), (128, 16384, -1, True)])
 def test_quicklook_matches_oracle(gp, m, n, sgn, window):
     """form_image_pfa.quicklook (window + one centred 2-D FFT of the raw history)."""
     sc, freq, ax, ay = _scene(m, n, seed=m + n)
@@ -127,6 +128,26 @@ def test_quicklook_refuses_other_sizes(gp):
     from mi355x_pfa._lib import PfaError
     with pytest.raises(PfaError, match="powers of two"):
         gp.quicklook(np.zeros((300, 256), np.complex64), -1)
+    with pytest.raises(PfaError, match="powers of two"):
+        gp.quicklook(np.zeros((16384, 256), np.complex64), -1)
+
+
+def test_padded_size_limits(gp):
+    """Padded range (sample) size up to 16384, padded cross (pulse) size up to
+    8192; beyond either the plan is refused with a dimension error."""
+    from mi355x_pfa._lib import PfaError
+
+    def plan(m, n):
+        geom = {"pr": np.ones(m), "order": np.arange(m, dtype=np.int32), "tan_s": np.zeros(m),
+                "KR": np.linspace(1.0, 2.0, n), "KC": np.linspace(-1.0, 1.0, m), "geo": {}}
+        return gp.FloatPlan(m, n, -1, geom, f0=np.ones(m), df=np.ones(m))
+    p = plan(100, 16384)
+    assert (p.m2, p.n2) == (128, 16384)
+    p.close()
+    with pytest.raises(PfaError, match="range"):
+        plan(100, 16385)
+    with pytest.raises(PfaError, match="cross"):
+        plan(8193, 100)
 
 
 @pytest.mark.parametrize("desc", [False, True])
