@@ -1,4 +1,5 @@
-"""The float focus at the benchmark's full size, 8167 x 8192 -> 8192^2, checked
+"""The float focus at full size, 8167 x 8192 -> 8192^2 (the benchmark frame) and
+5634 x 4319 -> 8192 x 8192 (the board scene's shape), checked
 through properties that do not need an oracle run of that size (the numpy
 oracle takes minutes per frame): linearity of the whole chain, point targets at
 the pixels the geometry predicts, the matrix-core FFT forms within their bound
@@ -15,14 +16,15 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module")
-def frame():
+@pytest.fixture(scope="module", params=["C3", "C2"])
+def frame(request):
     import torch
     from mi355x_pfa import synth
     from mi355x_pfa.pfa import float_geometry
     # targets within 40 m of the scene centre: the polar format's own geometric
     # distortion, about r^2 / (2 R) at 10 km standoff, stays below 0.2 pixel
-    spec = dataclasses.replace(synth.C3, targets=8, clutter=0, scene_half=40.0, seed=11)
+    base = {"C3": synth.C3, "C2": synth.C2}[request.param]
+    spec = dataclasses.replace(base, targets=8, clutter=0, scene_half=40.0, seed=11)
     print(f"full-size frame {spec.pulses}x{spec.samples}", flush=True)
     sc = synth.make_scene(spec, device="cuda")
     ax, ay = sc.look()
