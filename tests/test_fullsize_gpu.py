@@ -112,3 +112,52 @@ def test_fullsize_detect_equals_abs_of_complex(frame):
     d = _focus(plan, sc.sig, detect=True)
     assert torch.allclose(d, c.abs(), rtol=2e-6, atol=1e-6 * float(d.max()))
     plan.close()
+
+
+def test_fullsize_fixed_targets_match_float(frame):
+    """The fixed datapath (quantised int16 history, Q15 tapers, BFP FFTs) puts
+    every target where the float focus puts it.  The fixed image is
+    [range][cross] with no centring: its pixel (j, k) is the float image's
+    ((k - m2/2) mod m2, (j - n2/2) mod n2).  Peaks agree within one pixel."""
+    import torch
+    from mi355x_pfa import _lib
+    from mi355x_pfa.emulator import FixedPlan
+    from mi355x_pfa.tables import fixed_tables
+    import ctypes as C
+    spec, sc, geom = frame
+    m, n, G = spec.pulses, spec.samples, 8192
+    plan_f = _plan(frame, interp="linear")
+    a = _focus(plan_f, sc.sig, detect=True).cpu().numpy()
+    freq2 = np.zeros((m, n))
+    freq2[:, 0], freq2[:, 1] = sc.sc0, sc.sc0 + sc.scss
+    ax, ay = sc.look()
+    tabs = fixed_tables({"ax": ax, "ay": ay, "freq": freq2, "KR": geom["KR"], "KC": geom["KC"],
+                         "tan_phi": geom["tan_phi"]}, m, n, G)
+    plan_x = FixedPlan(tabs, m, n, G)
+    sig_iq = torch.empty((m, n, 2), dtype=torch.int16, device="cuda")
+    e = C.c_int(0)
+    torch.cuda.synchronize()
+    plan_x.ctx.check(_lib.lib().pfa_quantize_iq16_dev(plan_x.ctx.handle, sc.sig.data_ptr(), m * n,
+                                                      sig_iq.data_ptr(), C.byref(e)), "quantize")
+    out = torch.empty((G, G), dtype=torch.int16, device="cuda")
+    plan_x.form_image_dev(sig_iq, out)
+    plan_x.ctx.synchronize()
+    fx = out.view(torch.uint16).to(torch.int32).cpu().numpy()
+    m2, n2 = plan_f.m2, plan_f.n2
+    # fixed [range j][cross k] -> float layout [cross r][range c]
+    fx_c = np.roll(np.roll(fx.T, -m2 // 2, axis=0), -n2 // 2, axis=1)
+    geo = plan_f.padded_geo()
+    dh, ch = np.array(geo["dhat"]), np.array(geo["chat"])
+    for p in sc.targets:
+        r0 = int(round(m2 // 2 - (p[:2] @ ch) / geo["dc"]))
+        c0 = int(round(n2 // 2 - (p[:2] @ dh) / geo["dr"]))
+        wa = a[r0 - 2:r0 + 3, c0 - 2:c0 + 3]
+        wf = fx_c[r0 - 2:r0 + 3, c0 - 2:c0 + 3]
+        pa = np.unravel_index(np.argmax(wa), wa.shape)
+        pf = np.unravel_index(np.argmax(wf), wf.shape)
+        print(f"target near ({r0}, {c0}): float peak {pa}, fixed peak {pf}, fixed max {wf.max()}",
+              flush=True)
+        assert wf.max() > 0
+        assert abs(pa[0] - pf[0]) <= 1 and abs(pa[1] - pf[1]) <= 1
+    plan_f.close()
+    plan_x.close()
