@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <algorithm>
 #include <cstring>
 
 #include "pfa_internal.hpp"
@@ -20,49 +21,60 @@
 namespace {
 
 // ---------------------------------------------------------------- histogram
-// One workgroup counts HCHUNK values into 65536 16-bit counters packed two
-// per LDS word (128 KB); HCHUNK <= 65535 keeps a counter from carrying into
-// its neighbour.  The non-zero counters are then added to the global
-// histogram.
+// Persistent: one 1024-thread workgroup per CU keeps 32-bit counters for the
+// values below 32768 in LDS (128 KB) over its whole share of the image and
+// adds them to the global histogram once at the end.  Values of 32768 and
+// above (the cores of bright targets in a detected image) are counted with
+// global atomics directly.
 constexpr int HT = 1024;                  // threads per workgroup
-constexpr int HCHUNK = 32768;             // values per workgroup
-constexpr int HWORDS = 32768;             // LDS words: 65536 packed counters
-static_assert(HCHUNK <= 65535, "16-bit counters");
-static_assert(HCHUNK % (HT * 8) == 0, "whole 16-byte loads per thread");
+constexpr int HLDS = 32768;               // values counted in LDS
 
 template <bool VEC>
 __global__ __launch_bounds__(HT) void k_hist_u16(const uint16_t *__restrict__ img, int64_t count,
                                                  uint32_t *__restrict__ hist) {
-    __shared__ __align__(16) uint32_t h[HWORDS];
+    __shared__ __align__(16) uint32_t h[HLDS];
     const int t = threadIdx.x;
-    for (int w = t * 4; w < HWORDS; w += HT * 4) *reinterpret_cast<uint4 *>(&h[w]) = make_uint4(0, 0, 0, 0);
+    for (int w = t * 4; w < HLDS; w += HT * 4) *reinterpret_cast<uint4 *>(&h[w]) = make_uint4(0, 0, 0, 0);
     __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * HCHUNK;
-    auto add = [&](unsigned v) { atomicAdd(&h[v >> 1], 1u << ((v & 1u) << 4)); };
-    if (VEC && base + HCHUNK <= count) {
-        constexpr int NV = HCHUNK / (HT * 8);
-        uint4 d[NV];
+    auto add = [&](unsigned v) {
+        if (v < (unsigned)HLDS) atomicAdd(&h[v], 1u);
+        else atomicAdd(&hist[v], 1u);
+    };
+    const int64_t stride = (int64_t)gridDim.x * HT;
+    int64_t i = (int64_t)blockIdx.x * HT + t;
+    if (VEC) {
+        const int64_t n8 = count / 8;
+        const uint4 *img8 = reinterpret_cast<const uint4 *>(img);
+        for (; i + 3 * stride < n8; i += 4 * stride) {   // four 16-byte loads in flight
+            uint4 d[4];
 #pragma unroll
-        for (int k = 0; k < NV; ++k)
-            d[k] = *reinterpret_cast<const uint4 *>(img + base + ((int64_t)k * HT + t) * 8);
+            for (int k = 0; k < 4; ++k) d[k] = img8[i + k * stride];
 #pragma unroll
-        for (int k = 0; k < NV; ++k) {
-            const uint32_t w4[4] = {d[k].x, d[k].y, d[k].z, d[k].w};
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t w4[4] = {d[k].x, d[k].y, d[k].z, d[k].w};
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    add(w4[q] & 0xFFFFu);
+                    add(w4[q] >> 16);
+                }
+            }
+        }
+        for (; i < n8; i += stride) {
+            const uint4 d = img8[i];
+            const uint32_t w4[4] = {d.x, d.y, d.z, d.w};
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 add(w4[q] & 0xFFFFu);
                 add(w4[q] >> 16);
             }
         }
-    } else {
-        const int64_t end = base + HCHUNK < count ? base + HCHUNK : count;
-        for (int64_t i = base + t; i < end; i += HT) add(img[i]);
+        i = n8 * 8 + (int64_t)blockIdx.x * HT + t;
     }
+    for (; i < count; i += stride) add(img[i]);
     __syncthreads();
-    for (int w = t; w < HWORDS; w += HT) {
+    for (int w = t; w < HLDS; w += HT) {
         const uint32_t c = h[w];
-        if (c & 0xFFFFu) atomicAdd(&hist[2 * w], c & 0xFFFFu);
-        if (c >> 16) atomicAdd(&hist[2 * w + 1], c >> 16);
+        if (c) atomicAdd(&hist[w], c);
     }
 }
 
@@ -163,8 +175,10 @@ __global__ __launch_bounds__(ST) void k_stretch_u8(const float *__restrict__ x, 
 // Radix select on the order-preserving key of the float bits (NaN sorted
 // last, as np.partition / np.sort place it): three histogram passes over the
 // key's bits 31..21, 20..10 and 9..0, each restricted to the values whose
-// higher bits equal the prefix found so far, for two ranks at once.  A
-// one-workgroup scan between passes picks each rank's bucket on the device.
+// higher bits equal the prefix found so far, for two ranks at once.  One
+// histogram workgroup per CU keeps the flushes of its counters to the global
+// histogram few.  A one-workgroup scan between passes picks each rank's
+// bucket on the device.
 struct SelState {
     uint32_t prefix[2];                   // key bits found so far
     uint32_t pad[2];
@@ -173,7 +187,8 @@ struct SelState {
     uint32_t key[2];                      // final keys
 };
 constexpr int SEL_BINS = 2048;
-constexpr int SEL_T = 256;
+constexpr int SEL_T = 256;                // scan workgroup
+constexpr int SEL_HT = 1024;              // histogram workgroups: one per CU
 
 __device__ __forceinline__ uint32_t fkey(float f) {
     const uint32_t b = __float_as_uint(f);
@@ -196,12 +211,12 @@ __global__ void k_sel_init(SelState *st, unsigned long long k0, unsigned long lo
 // level 1: (key >> 10) & 2047 where key >> 21 == prefix[r];
 // level 2: key & 1023 where key >> 10 == prefix[r].
 template <int LEVEL, bool VEC>
-__global__ __launch_bounds__(SEL_T) void k_sel_hist(const float *__restrict__ x, int64_t count,
-                                                    const SelState *__restrict__ st,
-                                                    uint32_t *__restrict__ hist) {
+__global__ __launch_bounds__(SEL_HT) void k_sel_hist(const float *__restrict__ x, int64_t count,
+                                                     const SelState *__restrict__ st,
+                                                     uint32_t *__restrict__ hist) {
     __shared__ uint32_t h[2][SEL_BINS];
     const int t = threadIdx.x;
-    for (int i = t; i < 2 * SEL_BINS; i += SEL_T) h[i / SEL_BINS][i % SEL_BINS] = 0;
+    for (int i = t; i < 2 * SEL_BINS; i += SEL_HT) h[i / SEL_BINS][i % SEL_BINS] = 0;
     const uint32_t p0 = LEVEL ? st->prefix[0] : 0u, p1 = LEVEL ? st->prefix[1] : 0u;
     const bool two = LEVEL && p0 != p1;
     __syncthreads();
@@ -219,22 +234,36 @@ __global__ __launch_bounds__(SEL_T) void k_sel_hist(const float *__restrict__ x,
             else if (two && p == p1) atomicAdd(&h[1][b], 1u);
         }
     };
-    const int64_t stride = (int64_t)gridDim.x * SEL_T;
+    const int64_t stride = (int64_t)gridDim.x * SEL_HT;
     if (VEC) {
         const int64_t n4 = count / 4;
-        for (int64_t i = (int64_t)blockIdx.x * SEL_T + t; i < n4; i += stride) {
-            const float4 v = reinterpret_cast<const float4 *>(x)[i];
+        const float4 *x4 = reinterpret_cast<const float4 *>(x);
+        int64_t i = (int64_t)blockIdx.x * SEL_HT + t;
+        for (; i + 3 * stride < n4; i += 4 * stride) {    // four 16-byte loads in flight
+            float4 v[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) v[k] = x4[i + k * stride];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                add(v[k].x);
+                add(v[k].y);
+                add(v[k].z);
+                add(v[k].w);
+            }
+        }
+        for (; i < n4; i += stride) {
+            const float4 v = x4[i];
             add(v.x);
             add(v.y);
             add(v.z);
             add(v.w);
         }
-        for (int64_t i = n4 * 4 + (int64_t)blockIdx.x * SEL_T + t; i < count; i += stride) add(x[i]);
+        for (int64_t j = n4 * 4 + (int64_t)blockIdx.x * SEL_HT + t; j < count; j += stride) add(x[j]);
     } else {
-        for (int64_t i = (int64_t)blockIdx.x * SEL_T + t; i < count; i += stride) add(x[i]);
+        for (int64_t i = (int64_t)blockIdx.x * SEL_HT + t; i < count; i += stride) add(x[i]);
     }
     __syncthreads();
-    for (int i = t; i < 2 * SEL_BINS; i += SEL_T) {
+    for (int i = t; i < 2 * SEL_BINS; i += SEL_HT) {
         const uint32_t c = h[i / SEL_BINS][i % SEL_BINS];
         if (c) atomicAdd(&hist[i], c);
     }
@@ -313,11 +342,12 @@ float key_to_float(uint32_t k) {
 static int launch_hist_u16(pfa_ctx *ctx, const uint16_t *img, int64_t count, uint32_t *hist) {
     hipLaunchKernelGGL(k_fill_u32, dim3(65536 / 256), dim3(256), 0, ctx->stream, hist, 65536, 0u);
     if (count > 0) {
-        const int64_t wg = (count + HCHUNK - 1) / HCHUNK;
+        const int64_t need = (count + (int64_t)HT * 64 - 1) / ((int64_t)HT * 64);
+        const unsigned wg = (unsigned)std::max<int64_t>(1, std::min<int64_t>(need, ctx->num_cu));
         if (aligned16(img))
-            hipLaunchKernelGGL(k_hist_u16<true>, dim3((unsigned)wg), dim3(HT), 0, ctx->stream, img, count, hist);
+            hipLaunchKernelGGL(k_hist_u16<true>, dim3(wg), dim3(HT), 0, ctx->stream, img, count, hist);
         else
-            hipLaunchKernelGGL(k_hist_u16<false>, dim3((unsigned)wg), dim3(HT), 0, ctx->stream, img, count, hist);
+            hipLaunchKernelGGL(k_hist_u16<false>, dim3(wg), dim3(HT), 0, ctx->stream, img, count, hist);
     }
     PFA_HIP(ctx, hipGetLastError());
     return PFA_OK;
@@ -354,13 +384,14 @@ static int launch_stretch_u8(pfa_ctx *ctx, const float *x, int64_t count, float 
 
 static int launch_select_f32(pfa_ctx *ctx, const float *x, int64_t count, int64_t k0, int64_t k1,
                              SelState *st, uint32_t *hist) {
-    const unsigned g = stream_blocks(ctx, count, SEL_T, 64);
+    const int64_t need = (count + (int64_t)SEL_HT * 64 - 1) / ((int64_t)SEL_HT * 64);
+    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(need, ctx->num_cu));
     const bool vec = aligned16(x);
     hipLaunchKernelGGL(k_sel_init, dim3(1), dim3(SEL_T), 0, ctx->stream, st, (unsigned long long)k0,
                        (unsigned long long)k1, hist);
 #define SEL_LEVEL(L)                                                                                  \
-    if (vec) hipLaunchKernelGGL((k_sel_hist<L, true>), dim3(g), dim3(SEL_T), 0, ctx->stream, x, count, st, hist); \
-    else hipLaunchKernelGGL((k_sel_hist<L, false>), dim3(g), dim3(SEL_T), 0, ctx->stream, x, count, st, hist); \
+    if (vec) hipLaunchKernelGGL((k_sel_hist<L, true>), dim3(g), dim3(SEL_HT), 0, ctx->stream, x, count, st, hist); \
+    else hipLaunchKernelGGL((k_sel_hist<L, false>), dim3(g), dim3(SEL_HT), 0, ctx->stream, x, count, st, hist); \
     hipLaunchKernelGGL(k_sel_scan<L>, dim3(1), dim3(SEL_T), 0, ctx->stream, st, hist);
     SEL_LEVEL(0)
     SEL_LEVEL(1)
