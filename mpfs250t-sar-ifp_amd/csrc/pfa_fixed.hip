@@ -259,25 +259,41 @@ __device__ __forceinline__ short2 lerp_q15(const short2 *row, int idx, int wq, b
 // 32-tap polyphase sinc (gen_resample_vectors.gather_sinc): taps idx-15..idx+16,
 // exact integer dot product, round half up, sat16.  Each sample is split into
 // a signed high byte and an unsigned low byte so both partial sums stay in
-// int32 (|c*xh| <= 2^22, |c*xl| <= 2^23 over 32 taps).
-// The taps of one phase are loaded as four 16-byte words (a table row is 64
-// bytes, 64-byte aligned).
+// int32 (|c*xh| <= 2^22, |c*xl| <= 2^23 over 32 taps).  Two taps at a time:
+// the taps of a phase are int16 pairs as stored (a table row is 64 bytes,
+// 64-byte aligned, read as four 16-byte words); one byte permute builds each
+// of the two samples' high-byte or low-byte pairs as int16 lanes, and a packed
+// 16-bit dot product with 32-bit accumulation adds both products.
+__device__ __forceinline__ unsigned byte_pair(unsigned s1, unsigned s0, unsigned sel) {
+    return __builtin_amdgcn_perm(s1, s0, sel);
+}
+// selectors over {s1 : s0} (bytes 0..3 = s0 = (re0, im0), 4..7 = s1 = (re1, im1));
+// 8, 9, 10, 11 give the sign of bytes 1, 3, 5, 7 and 12 gives zero
+constexpr unsigned SEL_RE_HI = 0x0A050801u;   // (re0 >> 8, re1 >> 8)
+constexpr unsigned SEL_RE_LO = 0x0C040C00u;   // (re0 & 255, re1 & 255)
+constexpr unsigned SEL_IM_HI = 0x0B070903u;   // (im0 >> 8, im1 >> 8)
+constexpr unsigned SEL_IM_LO = 0x0C060C02u;   // (im0 & 255, im1 & 255)
+
+__device__ __forceinline__ int dot2(unsigned a, unsigned b, int c) {
+    typedef short v2s __attribute__((ext_vector_type(2)));
+    return __builtin_amdgcn_sdot2(__builtin_bit_cast(v2s, a), __builtin_bit_cast(v2s, b), c, false);
+}
+
 __device__ __forceinline__ short2 sinc_q15w(const short2 *row, int idx, const int16_t *taps) {
     const uint4 *tw = reinterpret_cast<const uint4 *>(taps);
     uint4 w[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) w[i] = tw[i];
     int hr = 0, lr = 0, hi = 0, li = 0;
-    const short2 *s = row + idx - 15;
+    const unsigned *s = reinterpret_cast<const unsigned *>(row + idx - 15);
 #pragma unroll
-    for (int t = 0; t < 32; ++t) {
-        const unsigned word = (&w[t >> 3].x)[(t >> 1) & 3];
-        const int c = (t & 1) ? ((int)word >> 16) : (int)(short)(word & 0xffffu);
-        const short2 v = s[t];
-        hr += c * (v.x >> 8);
-        lr += c * (v.x & 255);
-        hi += c * (v.y >> 8);
-        li += c * (v.y & 255);
+    for (int k = 0; k < 16; ++k) {
+        const unsigned cw = (&w[k >> 2].x)[k & 3];    // taps 2k (low half), 2k + 1
+        const unsigned s0 = s[2 * k], s1 = s[2 * k + 1];
+        hr = dot2(cw, byte_pair(s1, s0, SEL_RE_HI), hr);
+        lr = dot2(cw, byte_pair(s1, s0, SEL_RE_LO), lr);
+        hi = dot2(cw, byte_pair(s1, s0, SEL_IM_HI), hi);
+        li = dot2(cw, byte_pair(s1, s0, SEL_IM_LO), li);
     }
     const int64_t ar = (int64_t)hr * 256 + lr + 16384;
     const int64_t ai = (int64_t)hi * 256 + li + 16384;
