@@ -150,14 +150,21 @@ __device__ __forceinline__ void pass(float2 *buf, const Tw &tw, Src *src, Sink &
     int t = threadIdx.x;
     asm volatile("" : "+v"(t));
     t &= 1023;                                         // known non-negative: k / LS, k % LS are shifts
+    // For power-of-two N, R and LS the padded index splits exactly:
+    // padf(k + r N/R) = padf(k) + r N/R + (r N/R) / 32 for k < N/R, and
+    // padf(o + r LS) = padf(o) + r LS + (r LS) / 32 for the outputs o of a
+    // butterfly, so every element of a butterfly sits at a compile-time offset
+    // from one base address (immediate offsets of the LDS instructions).
+    constexpr int NR = N / R;
     float2 x[B][R];
     if (t < NT) {
 #pragma unroll
         for (int b = 0; b < B; ++b) {
             const int k = t + b * NT;
+            const float2 *rb = buf + padf(k);
 #pragma unroll
             for (int r = 0; r < R; ++r)
-                x[b][r] = src ? call_src(*src, k + r * (N / R), b + r * B) : buf[padf(k + r * (N / R))];
+                x[b][r] = src ? call_src(*src, k + r * NR, b + r * B) : rb[r * NR + (r * NR) / 32];
         }
     }
     if (!src) __syncthreads();
@@ -177,10 +184,11 @@ __device__ __forceinline__ void pass(float2 *buf, const Tw &tw, Src *src, Sink &
             }
             dft<R, INV>(x[b]);
             const int o = (k / LS) * LS * R + j;
+            float2 *wb = buf + padf(o);
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 if (LAST) sink(o + r * LS, x[b][r]);
-                else buf[padf(o + r * LS)] = x[b][r];
+                else wb[r * LS + (r * LS) / 32] = x[b][r];
             }
         }
     }
