@@ -89,12 +89,18 @@ __global__ __launch_bounds__(256) void k_af_stats2(const double *__restrict__ pa
 
 // Adjoint of FFT-B + output sign fold for image row r.  INVF: the forward
 // transform was the inverse one (sgn > 0, with its 1/N).  Output B[r][j], j < n.
+// 16 elements per thread (radix-16 passes): twice the threads of the default
+// configuration, so a CU holds enough waves to cover the row loads.
+constexpr int AF_FB_EMAX = 16;
+template <int L>
+constexpr int af_fb_nb() { return pfa32::Cfg<L, AF_FB_EMAX>::NT < 64 ? 64 : pfa32::Cfg<L, AF_FB_EMAX>::NT; }
+
 template <int L, bool INVF>
-__global__ __launch_bounds__(nb_of<L>()) void k_af_fftb_adj(const float2 *__restrict__ img, int n,
-                                                            const double *__restrict__ stats,
-                                                            float2 *__restrict__ B,
-                                                            const float2 *__restrict__ twg) {
-    using Cf = pfa32::Cfg<L>;
+__global__ __launch_bounds__(af_fb_nb<L>()) void k_af_fftb_adj(const float2 *__restrict__ img, int n,
+                                                               const double *__restrict__ stats,
+                                                               float2 *__restrict__ B,
+                                                               const float2 *__restrict__ twg) {
+    using Cf = pfa32::Cfg<L, AF_FB_EMAX>;
     constexpr int N = Cf::N;
     __shared__ float2 buf[Cf::PADN];
     __shared__ float2 twl[N / 64 + 64];
@@ -112,7 +118,7 @@ __global__ __launch_bounds__(nb_of<L>()) void k_af_fftb_adj(const float2 *__rest
         if ((r + c) & 1) v = make_float2(-v.x, -v.y);
         return v;
     };
-    pfa32::fft_src<L, !INVF>(buf, tw, src, [&](int j, float2 v) {
+    pfa32::fft_src<L, !INVF, AF_FB_EMAX>(buf, tw, src, [&](int j, float2 v) {
         if (j < n) B[(size_t)r * n + j] = make_float2(v.x * scale, v.y * scale);
     });
 }
@@ -327,9 +333,9 @@ int launch_af_fftb_adj(pfa_plan *p, const void *img, const double *stats, void *
     const int L = log2_exact(p->n2);
 #define LAUNCH(LL)                                                                                  \
     case LL:                                                                                        \
-        if (inv) hipLaunchKernelGGL((k_af_fftb_adj<LL, true>), dim3(p->m2), dim3(nb_of<LL>()), 0,   \
+        if (inv) hipLaunchKernelGGL((k_af_fftb_adj<LL, true>), dim3(p->m2), dim3(af_fb_nb<LL>()), 0, \
                                     ctx->stream, (const float2 *)img, p->n, stats, (float2 *)B, tw); \
-        else hipLaunchKernelGGL((k_af_fftb_adj<LL, false>), dim3(p->m2), dim3(nb_of<LL>()), 0,      \
+        else hipLaunchKernelGGL((k_af_fftb_adj<LL, false>), dim3(p->m2), dim3(af_fb_nb<LL>()), 0,    \
                                 ctx->stream, (const float2 *)img, p->n, stats, (float2 *)B, tw);     \
         break;
     PFA_AF_SWITCH_N(L, LAUNCH)
