@@ -449,7 +449,20 @@ __global__ __launch_bounds__(RT, 2) void k_range_f32(
 // ------------------------------------------------------------------ pass 2 (+ window + FFT A)
 // A is row-major [range bin j][cross r]: every workgroup of the azimuth
 // kernel writes whole 128-byte lines.
-__device__ __forceinline__ size_t ablk(int j, int r, int m2) { return (size_t)j * m2 + r; }
+// A, the azimuth pass's output [range row j][cross bin r], is stored in blocks
+// of 8 cross bins: element (j, r) at ((r / 8) nj + j) 8 + r % 8, with nj = the
+// row count rounded up to 16.  A line's FFT writes whole 64-byte chunks
+// (8 consecutive r), and the column pairs FFT-B reads lie 64 bytes apart for
+// consecutive rows, so a wave's 16-byte loads touch 64-byte chunks of 32
+// cache lines instead of one line per row.  Measured at 8192^2 (identical
+// images): 16-bin blocks 0.302 ms FFT-B, 8-bin 0.239 ms with the azimuth pass
+// unchanged, 4-bin 0.209 ms FFT-B but 1.16 ms azimuth (its 32-byte scattered
+// writes), row-major 0.316 ms.
+constexpr int A_BLK = 8;
+__device__ __forceinline__ int a_rows(int n) { return (n + 15) & ~15; }
+__device__ __forceinline__ size_t ablk(int j, int r, int nj) {
+    return ((size_t)(r / A_BLK) * nj + j) * A_BLK + (r % A_BLK);
+}
 
 // Workgroup b -> consecutive items on the same XCD (b % 8 selects the XCD).
 __device__ __forceinline__ int xcd_map(int b, int count) {
@@ -487,9 +500,9 @@ struct TanBins {
 // previous line) and the windowed, sign-folded line the FFT runs on.
 // Thread t owns outputs 4 (g NB + t) + u, u < 4: the four share one sinc window.
 // MODE 0: store the windowed k-space g2[q][j] (resample_kspace stage output)
-// MODE 1: fused -- window, sign fold, pad, FFT along cross -> A (row-major)
+// MODE 1: fused -- window, sign fold, pad, FFT along cross -> A (8-bin blocks, ablk)
 // MODE 2: FFT along cross of a given g2 (focus() stage; windowed when the plan has tapers:
-//         quicklook()) -> A (row-major)
+//         quicklook()) -> A (8-bin blocks, ablk)
 // FFT form of the line transform: 0 FP32 vector FFT (A complex64); 1 BF16,
 // 2 FP16, 3 BF16 with hi/lo DFT matrices -- matrix-core FFT (pfa_fft_mfma.hpp),
 // A stored as packed 16-bit pairs with a per-row exponent.
@@ -772,11 +785,11 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
         }
         if constexpr (MODE != 0) {
             __syncthreads();
-            float2 *dst = out + ablk(j, 0, M2);
+            const int nj = a_rows(n);
             const float scale = INV ? 1.0f / (float)M2 : 1.0f;
             pfa32::fft<L, INV, AZ_FFT_EMAX>(reg, tw, [&](int idx, float2 w) {
                 if (INV) w = make_float2(w.x * scale, w.y * scale);
-                st_stream(dst + idx, w);
+                st_stream(out + ablk(j, idx, nj), w);
             });
         }
     }
@@ -807,13 +820,14 @@ __global__ __launch_bounds__(fb2_nb<L>(), 4) void k_fft_b2_f32(
     const pfa32::Tw tw = pfa32::load_tw<N2>(twl, twg);
     const int r0 = 2 * xcd_map(blockIdx.x, m2 / 2);
     const int t = threadIdx.x;
+    const int nj = a_rows(n);
     const float scale = INV ? 1.0f / (float)N2 : 1.0f;
     float2 c0[E], c1[E];
 #pragma unroll
     for (int s2 = 0; s2 < E; ++s2) {
         const int jj = t + s2 * NT;
         float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (t < NT && jj < n) v = *reinterpret_cast<const float4 *>(A + ablk(jj, r0, m2));
+        if (t < NT && jj < n) v = *reinterpret_cast<const float4 *>(A + ablk(jj, r0, nj));
         c0[s2] = make_float2(v.x, v.y);
         c1[s2] = make_float2(v.z, v.w);
     }
