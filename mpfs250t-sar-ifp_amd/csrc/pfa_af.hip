@@ -10,7 +10,7 @@
 // load.  With G = dS/dI* = (2 I / E2^2)(|I|^2 - E4 / E2) (Wirtinger), the
 // backward is
 //   k_af_fftb_adj   per image row r: conjugate-direction FFT along range of
-//                   the sign-folded G -> B[r][j]
+//                   the sign-folded G -> B[r][j] (8-bin blocks along j)
 //   k_af_az_adj     per range bin j: conjugate-direction FFT along cross of
 //                   column j of B (the corner-turn in its load), sign fold,
 //                   window, then the transpose of the azimuth gather as a
@@ -87,8 +87,17 @@ __global__ __launch_bounds__(256) void k_af_stats2(const double *__restrict__ pa
     }
 }
 
-// Adjoint of FFT-B + output sign fold for image row r.  INVF: the forward
-// transform was the inverse one (sgn > 0, with its 1/N).  Output B[r][j], j < n.
+// B, the adjoint of FFT-B [cross row r][range bin j], is stored in blocks of 8
+// range bins: element (r, j) at ((j / 8) m2 + r) 8 + j % 8.  A row's transform
+// writes 64-byte chunks, and the azimuth adjoint's reads of one range bin down
+// consecutive rows are 64 bytes apart rather than a full row (n * 8 bytes).
+__device__ __forceinline__ size_t bblk(int r, int j, int m2) {
+    return ((size_t)(j >> 3) * m2 + r) * 8 + (j & 7);
+}
+
+// Adjoint of FFT-B + output sign fold for image row r (one workgroup per row,
+// gridDim.x = m2).  INVF: the forward transform was the inverse one (sgn > 0,
+// with its 1/N).  Output B (bblk layout), j < n.
 // 16 elements per thread (radix-16 passes): twice the threads of the default
 // configuration, so a CU holds enough waves to cover the row loads.
 constexpr int AF_FB_EMAX = 16;
@@ -119,7 +128,7 @@ __global__ __launch_bounds__(af_fb_nb<L>()) void k_af_fftb_adj(const float2 *__r
         return v;
     };
     pfa32::fft_src<L, !INVF, AF_FB_EMAX>(buf, tw, src, [&](int j, float2 v) {
-        if (j < n) B[(size_t)r * n + j] = make_float2(v.x * scale, v.y * scale);
+        if (j < n) B[bblk(r, j, gridDim.x)] = make_float2(v.x * scale, v.y * scale);
     });
 }
 
@@ -197,7 +206,7 @@ __global__ __launch_bounds__(az_nb<L>()) void k_af_az_adj(
         }
         __syncthreads();                               // positions dead: the FFT line takes over
         const double hj = hn ? hn[j] : 1.0;
-        pfa32::fft_src<L, !INVF, EM>(reg, tw, [&](int r) { return B[(size_t)r * n + j]; },
+        pfa32::fft_src<L, !INVF, EM>(reg, tw, [&](int r) { return B[bblk(r, j, M2)]; },
             [&](int q, float2 v) {
                 if (q < m) {
                     float w = scale;
@@ -282,15 +291,28 @@ __global__ __launch_bounds__(az_nb<L>()) void k_af_az_adj(
 }
 
 // grad[order[p]] = (2 / E2) sum_b partial[b][p]   (the 1/E2 undoes G's scaling)
-__global__ __launch_bounds__(256) void k_af_grad(const float *__restrict__ partial, int nparts, int m,
-                                                 const int32_t *__restrict__ order,
-                                                 const double *__restrict__ stats,
-                                                 float *__restrict__ grad) {
-    const int p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= m) return;
+// 32 sources per workgroup, their sums split over 8 slices of the partials
+// (coalesced 128-byte rows, 8 independent chains), combined in LDS in a fixed
+// order.
+constexpr int GR_P = 32, GR_S = 8;
+__global__ __launch_bounds__(GR_P * GR_S) void k_af_grad(const float *__restrict__ partial, int nparts,
+                                                         int m, const int32_t *__restrict__ order,
+                                                         const double *__restrict__ stats,
+                                                         float *__restrict__ grad) {
+    __shared__ double red[GR_S][GR_P];
+    const int pl = threadIdx.x % GR_P, sl = threadIdx.x / GR_P;
+    const int p = blockIdx.x * GR_P + pl;
     double s = 0.0;
-    for (int b = 0; b < nparts; ++b) s += partial[(size_t)b * m + p];
-    grad[order[p]] = stats[0] > 0.0 ? (float)(2.0 * s / stats[0]) : 0.f;
+    if (p < m)
+        for (int b = sl; b < nparts; b += GR_S) s += partial[(size_t)b * m + p];
+    red[sl][pl] = s;
+    __syncthreads();
+    if (sl == 0 && p < m) {
+        double tot = 0.0;
+#pragma unroll
+        for (int k = 0; k < GR_S; ++k) tot += red[k][pl];
+        grad[order[p]] = stats[0] > 0.0 ? (float)(2.0 * tot / stats[0]) : 0.f;
+    }
 }
 
 }  // namespace
@@ -378,8 +400,8 @@ int launch_af_az_adj(pfa_plan *p, const void *g1t, int ldp, const void *rot, con
 int launch_af_grad_reduce(pfa_plan *p, const float *partial, int nparts, const double *stats,
                           float *grad) {
     pfa_ctx *ctx = p->ctx;
-    hipLaunchKernelGGL(k_af_grad, dim3((p->m + 255) / 256), dim3(256), 0, ctx->stream, partial,
-                       nparts, p->m, p->order, stats, grad);
+    hipLaunchKernelGGL(k_af_grad, dim3((p->m + GR_P - 1) / GR_P), dim3(GR_P * GR_S), 0, ctx->stream,
+                       partial, nparts, p->m, p->order, stats, grad);
     PFA_HIP(ctx, hipGetLastError());
     return PFA_OK;
 }

@@ -685,7 +685,7 @@ extern "C" int pfa_af_backward(pfa_plan *p, const float *phi, const float *img,
     int st = PFA_OK;
     void *g1t, *rot;
     CK(af_buffers(p, &g1t, &rot, phi));
-    void *B = pfa_ws(p->ctx, "af_B", (size_t)p->m2 * p->n * 8, &st);
+    void *B = pfa_ws(p->ctx, "af_B", (size_t)p->m2 * ((p->n + 7) & ~7) * 8, &st);   // 8-bin blocks
     if (!B) return st;
     const size_t max_parts = (size_t)std::max(p->ctx->num_cu, 8) + 8;   // >= the adjoint's grid
     float *part = (float *)pfa_ws(p->ctx, "af_part", max_parts * p->m * 4, &st);
