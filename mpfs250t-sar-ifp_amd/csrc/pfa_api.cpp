@@ -611,7 +611,7 @@ extern "C" int pfa_focus_f32_dev(pfa_plan *p, const float *sig, const double *fr
     const int fk = (p->flags & PFA_FLOAT_FFT_BF16) ? ((p->flags & PFA_FLOAT_FFT_SPLIT) ? 3 : 1)
                    : (p->flags & PFA_FLOAT_FFT_FP16) ? 2 : 0;
     if (fk) {                                  // precision form: A as 16-bit pairs + row exponents
-        unsigned *A16 = (unsigned *)pfa_ws(ctx, "f_A16", (size_t)p->n * p->m2 * 4, &st);
+        unsigned *A16 = (unsigned *)pfa_ws(ctx, "f_A16", (size_t)(p->n + 15) / 16 * 16 * p->m2 * 4, &st);
         if (!A16) return st;
         int *expA = (int *)pfa_ws(ctx, "f_expA", (size_t)p->n * 4, &st);
         if (!expA) return st;

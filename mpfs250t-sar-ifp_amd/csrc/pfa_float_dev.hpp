@@ -463,6 +463,11 @@ __device__ __forceinline__ int a_rows(int n) { return (n + 15) & ~15; }
 __device__ __forceinline__ size_t ablk(int j, int r, int nj) {
     return ((size_t)(r / A_BLK) * nj + j) * A_BLK + (r % A_BLK);
 }
+// The matrix-core forms' A16 (one 32-bit word per complex sample) in blocks of
+// 16 cross bins: the same 64-byte chunks.
+__device__ __forceinline__ size_t a16blk(int j, int r, int nj) {
+    return ((size_t)(r >> 4) * nj + j) * 16 + (r & 15);
+}
 
 // Workgroup b -> consecutive items on the same XCD (b % 8 selects the XCD).
 __device__ __forceinline__ int xcd_map(int b, int count) {
@@ -761,10 +766,10 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
                 }
                 __syncthreads();
                 constexpr int NW = pfam::nw_for<L, 1, NB / 64>();
-                unsigned *dst = A16 + (size_t)j * M2;
+                const int nj = a_rows(n);
                 pfam::fft<L, BF, INV, SPLIT, 1, NW>(planes, tw, pfam::OpsGlobal<L, BF, INV>{ops},
                     [&](int, int o, float2 y) {
-                        __builtin_nontemporal_store(pfam::pack<BF>(y), dst + o);
+                        __builtin_nontemporal_store(pfam::pack<BF>(y), A16 + a16blk(j, o, nj));
                     });
                 if (t == 0) expA[j] = se + (INV ? 0 : L);
                 continue;

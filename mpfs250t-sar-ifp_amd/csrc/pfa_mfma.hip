@@ -7,9 +7,9 @@
 //
 // Kernels
 //   k_azimuth_f32<.., FK>  azimuth pass + window + sign fold (FP32), then the
-//                          FFT along cross on the matrix cores -> A16[j][r], expA[j]
+//                          FFT along cross on the matrix cores -> A16 (16-bin blocks), expA[j]
 //   k_fft_b_mfma           FFT along range for 4 adjacent cross bins per
-//                          workgroup (16-byte row loads of A16 = the corner-turn),
+//                          workgroup (16-byte loads of A16 rows = the corner-turn),
 //                          output sign fold and detection in the store
 #include "pfa_float_dev.hpp"
 
@@ -42,21 +42,21 @@ __global__ __launch_bounds__(fftb_mfma_nb<L>()) void k_fft_b_mfma(
     const pfa32::Tw tw = pfa32::load_tw<N2>(twl, twg);
     const int t = threadIdx.x;
     const int r0 = xcd_map(blockIdx.x, m2 / NCOL) * NCOL;
-    // the corner-turn: 4 adjacent cross bins of every range row.  Lane t takes
-    // row k + r NV0 (r = t % R0): consecutive lanes write consecutive words of
-    // the first-stage layout (rows are 4 m2 bytes apart in A16, so the
-    // assignment does not change the global requests).  All loads are issued
-    // first (clamped rows); the row exponents loaded with them also give the
-    // block's common exponent.
-    constexpr int R0 = pfam::MPlan<L>::R[0], NV0 = N2 / R0, NI = N2 / NB;
-    static_assert(NB % R0 == 0 && N2 % NB == 0, "row assignment covers the line");
-    auto row_of = [&](int i) { return t / R0 + i * (NB / R0) + (t % R0) * NV0; };
+    // the corner-turn: 4 adjacent cross bins of every range row.  A16 is in
+    // 16-bin blocks, so consecutive rows of a block are 64 bytes apart and
+    // consecutive lanes take consecutive rows: a wave's 16-byte loads fall in
+    // 32 cache lines.  All loads are issued first (clamped rows); the row
+    // exponents loaded with them also give the block's common exponent.
+    constexpr int NI = N2 / NB;
+    static_assert(N2 % NB == 0, "row assignment covers the line");
+    const int nj = a_rows(n);
+    auto row_of = [&](int i) { return t + i * NB; };
     uint4 a[NI];
     int ex[NI];
 #pragma unroll
     for (int i = 0; i < NI; ++i) {
         const int jc = min(row_of(i), n - 1);
-        a[i] = *reinterpret_cast<const uint4 *>(A16 + (size_t)jc * m2 + r0);
+        a[i] = *reinterpret_cast<const uint4 *>(A16 + a16blk(jc, r0, nj));
         ex[i] = expA[jc];
     }
     // common exponent: |A16| < 2^2, so 2^-(max expA + 2) bounds every input by 1
