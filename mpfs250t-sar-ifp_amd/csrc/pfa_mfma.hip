@@ -17,8 +17,9 @@
 
 namespace {
 
+constexpr int FBM_NCOL = 2;                 // cross bins (lines) per FFT-B workgroup
 template <int L>
-constexpr int fftb_mfma_nb() { return 64 * pfam::nw_for<L, 4, 8>(); }
+constexpr int fftb_mfma_nb() { return 64 * pfam::nw_for<L, FBM_NCOL, 8>(); }
 
 // DFT-matrix operand table of an L-point matrix-core FFT (pfam::OpsGlobal layout).
 template <int L, bool BF, bool INV>
@@ -33,7 +34,7 @@ __global__ __launch_bounds__(fftb_mfma_nb<L>()) void k_fft_b_mfma(
     void *__restrict__ outp, const float2 *__restrict__ twg, const pfam::u16x8 *__restrict__ ops) {
     constexpr bool BF = FK == FK_BF16 || FK == FK_BF16S;
     constexpr bool SPLIT = FK == FK_BF16S;
-    constexpr int N2 = 1 << L, NCOL = 4, NB = fftb_mfma_nb<L>(), BW = NB / 64;
+    constexpr int N2 = 1 << L, NCOL = FBM_NCOL, NB = fftb_mfma_nb<L>(), BW = NB / 64;
     constexpr int NW = pfam::nw_for<L, NCOL, BW>();
     constexpr int PL = pfam::line_words<L>();
     __shared__ unsigned planes[NCOL * PL];
@@ -51,12 +52,19 @@ __global__ __launch_bounds__(fftb_mfma_nb<L>()) void k_fft_b_mfma(
     static_assert(N2 % NB == 0, "row assignment covers the line");
     const int nj = a_rows(n);
     auto row_of = [&](int i) { return t + i * NB; };
-    uint4 a[NI];
+    unsigned a[NI][NCOL];
     int ex[NI];
 #pragma unroll
     for (int i = 0; i < NI; ++i) {
         const int jc = min(row_of(i), n - 1);
-        a[i] = *reinterpret_cast<const uint4 *>(A16 + a16blk(jc, r0, nj));
+        const unsigned *src = A16 + a16blk(jc, r0, nj);
+        if constexpr (NCOL == 4) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(src);
+            a[i][0] = v.x; a[i][1] = v.y; a[i][2] = v.z; a[i][3] = v.w;
+        } else {
+            const uint2 v = *reinterpret_cast<const uint2 *>(src);
+            a[i][0] = v.x; a[i][1] = v.y;
+        }
         ex[i] = expA[jc];
     }
     // common exponent: |A16| < 2^2, so 2^-(max expA + 2) bounds every input by 1
@@ -75,10 +83,9 @@ __global__ __launch_bounds__(fftb_mfma_nb<L>()) void k_fft_b_mfma(
         const int jj = row_of(i);
         const bool in = jj < n;
         const float sc = ldexpf(1.0f, ex[i] - S);
-        const unsigned wv[4] = {a[i].x, a[i].y, a[i].z, a[i].w};
 #pragma unroll
         for (int c = 0; c < NCOL; ++c) {
-            const float2 v = pfam::unpack<BF>(wv[c]);
+            const float2 v = pfam::unpack<BF>(a[i][c]);
             pfam::put<L, BF>(planes, c, jj, in ? make_float2(v.x * sc, v.y * sc) : make_float2(0.f, 0.f));
         }
     }
@@ -181,7 +188,7 @@ int launch_float_fft_b_mfma(pfa_plan *p, const unsigned *A16, const int *expA, v
     const int L = log2_exact(n2);
     const pfam::u16x8 *ops = dft_ops(ctx, L, fk != FK_FP16, inv, &st);
     if (!ops) return st;
-    const dim3 grid(p->m2 / 4);
+    const dim3 grid(p->m2 / FBM_NCOL);
     pfa_timer_begin(ctx, PFA_TIMER_FFT_B);
 #define ARGS A16, expA, p->n, p->m2, out, tw, ops
 #define K(LL, I, O, F) hipLaunchKernelGGL((k_fft_b_mfma<LL, I, O, F>), grid, dim3(fftb_mfma_nb<LL>()), 0, ctx->stream, ARGS)
