@@ -153,6 +153,10 @@ __device__ __forceinline__ void dft_operand(int mt, int ks, int l, typename H16<
         if (br == bc) v = c;
         else if (br == 0) v = INV ? -s : s;            // (0, 1)
         else v = INV ? s : -s;                         // (1, 0)
+        // the stage's 1/R scale, folded in: a power of two, so the 16-bit
+        // entries and every FP32 product and sum are exactly 1/R of the
+        // unscaled ones (all entries stay far above the 16-bit normal range)
+        v *= 1.0f / (float)R;
         const typename H16<BF>::T h = (typename H16<BF>::T)v;
         hi[i] = h;
         lo[i] = (typename H16<BF>::T)(v - (float)h);
@@ -206,7 +210,7 @@ __device__ __forceinline__ void build_ops_stage(u16x8 *tab) {
 }
 
 // One stage over NCOL lines held in `planes` (line_words<L>() words per line).  Stage S reads its layout, applies the
-// DFT, scales by 1/R, then either (not last) multiplies by the next stage's
+// DFT (its 1/R scale folded into the operands), then either (not last) multiplies by the next stage's
 // twiddle and writes the next layout, or (last) hands (line, index, value) to
 // sink.  NW waves share the work; results wait in registers for the barrier.
 template <int L, bool BF, bool INV, bool SPLIT, int NCOL, int NW, int S, typename Ops, typename Sink>
@@ -258,7 +262,6 @@ __device__ __forceinline__ void stage(unsigned *planes, const pfa32::Tw &tw, con
         }
     }
     __syncthreads();                                   // every operand read before any write
-    constexpr float inv_r = 1.0f / (float)R;
 #pragma unroll
     for (int g = 0; g < GPW; ++g) {
         if (!active) break;
@@ -273,7 +276,7 @@ __device__ __forceinline__ void stage(unsigned *planes, const pfa32::Tw &tw, con
             for (int i = 0; i < 4; ++i) {
                 const int m = 16 * mt + 4 * q + i;     // output radix index
                 const int o = obase + m * LS;
-                float2 y = make_float2(acc[g][mt][i] * inv_r, acc[g][mt + MT / 2][i] * inv_r);
+                float2 y = make_float2(acc[g][mt][i], acc[g][mt + MT / 2][i]);   // 1/R is in the operands
                 if constexpr (LAST) {
                     sink(col, o, y);
                 } else {
