@@ -70,9 +70,15 @@ template <bool BF>
 __device__ __forceinline__ float from16(unsigned short b) {
     return (float)__builtin_bit_cast(typename H16<BF>::T, b);
 }
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+// both halves in one conversion (v_cvt_pk_bf16_f32 for BF16), round to nearest even
 template <bool BF>
 __device__ __forceinline__ unsigned pack(float2 v) {
-    return (unsigned)to16<BF>(v.x) | ((unsigned)to16<BF>(v.y) << 16);
+    const f32x2 x = {v.x, v.y};
+    if constexpr (BF) return __builtin_bit_cast(unsigned, __builtin_convertvector(x, bf16x2));
+    else return __builtin_bit_cast(unsigned, __builtin_convertvector(x, f16x2));
 }
 template <bool BF>
 __device__ __forceinline__ float2 unpack(unsigned w) {
@@ -265,32 +271,46 @@ __device__ __forceinline__ void stage(unsigned *planes, const pfa32::Tw &tw, con
 #pragma unroll
     for (int g = 0; g < GPW; ++g) {
         if (!active) break;
-        const int G = w + g * NW;
-        const int col = G / (NV / 16), k = (G % (NV / 16)) * 16 + n;
-        const int j = k % LS;
-        const int obase = (k / LS) * LS * R + j;
-        unsigned *pl = planes + col * PL;
+        const unsigned G = w + g * NW;
+        const unsigned col = G / (NV / 16), k = (G % (NV / 16)) * 16 + n;
+        const unsigned j = k % LS;
+        if constexpr (LAST) {
+            const unsigned obase = (k / LS) * LS * R + j;
 #pragma unroll
-        for (int mt = 0; mt < MT / 2; ++mt) {
+            for (int mt = 0; mt < MT / 2; ++mt)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int m = 16 * mt + 4 * q + i;     // output radix index
-                const int o = obase + m * LS;
-                float2 y = make_float2(acc[g][mt][i], acc[g][mt + MT / 2][i]);   // 1/R is in the operands
-                if constexpr (LAST) {
-                    sink(col, o, y);
-                } else {
-                    constexpr int R2 = MPlan<L>::R[S + 1];
-                    constexpr int NV2 = N / R2;
-                    constexpr int LS2 = LS * R;
-                    const int k2 = o % NV2, r2 = o / NV2;
-                    const int j2 = k2 % LS2;
-                    if (j2 * r2) {
-                        float2 wv = tw(j2 * r2 * (N / (LS2 * R2)));
-                        if (INV) wv.y = -wv.y;
-                        y = pfa32::cmul(y, wv);
-                    }
-                    pl[pp(k2 * R2 + r2)] = pack<BF>(y);
+                for (int i = 0; i < 4; ++i) {
+                    const unsigned m = 16 * mt + 4 * q + i;    // output radix index
+                    sink((int)col, (int)(obase + m * LS),
+                         make_float2(acc[g][mt][i], acc[g][mt + MT / 2][i]));   // 1/R is in the operands
+                }
+        } else {
+            // Output m = 16 mt + 4 q + i sits at o = (k / LS) LS R + j + m LS.  The
+            // next stage (radix R2, NV2 = N / R2 vectors, LS2 = LS R) reads it as
+            // vector k2 = o % NV2, radix index r2 = o / NV2, twiddle exponent
+            // (k2 % LS2) r2 N / (LS2 R2).  NV2 is a multiple of LS2 and m LS < LS2,
+            // so r2 is the lane's own and, over (mt, i), the word position
+            // k2 R2 + r2 and the exponent step by compile-time multiples:
+            //     position  = p0 + (16 mt + i) LS R2,  pp(p0 + d) = pp(p0) + pp(d)
+            //     exponent  = T0 + (16 mt + i) D,      D = LS r2 N / (LS2 R2)
+            // (no carry crosses the pad boundaries of pp; both identities checked
+            // exhaustively for every size and stage).  The twiddle of exponent 0
+            // is exactly (1, 0), so it is applied unconditionally.
+            constexpr int R2 = MPlan<L>::R[S + 1];
+            constexpr unsigned NV2 = N / R2, LS2 = LS * R, KB = NV2 / LS2, C = N / (LS2 * R2);
+            static_assert(NV2 % LS2 == 0, "the next stage's vectors hold whole twiddle classes");
+            const unsigned h = k / LS, r2 = h / KB;
+            const unsigned kq = j + 4 * q * LS;
+            unsigned *dst = planes + col * PL + pp(((h % KB) * LS2 + kq) * R2 + r2);
+            const unsigned T0 = kq * r2 * C, D = LS * r2 * C;
+#pragma unroll
+            for (int mt = 0; mt < MT / 2; ++mt) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    float2 wv = tw((int)(T0 + (16 * mt + i) * D));
+                    if (INV) wv.y = -wv.y;
+                    const float2 y = pfa32::cmul(make_float2(acc[g][mt][i], acc[g][mt + MT / 2][i]), wv);
+                    dst[pp((16 * mt + i) * LS * R2)] = pack<BF>(y);
                 }
             }
         }
