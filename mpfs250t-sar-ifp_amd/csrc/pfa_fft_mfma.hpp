@@ -127,8 +127,8 @@ constexpr int line_words() { return ((1 << L) + ((1 << L) >> 3) + (((1 << L) >> 
 
 // Position of element e in the plane layout read by stage S.
 template <int L, int S>
-__device__ __forceinline__ int pos_of(int e) {
-    constexpr int R = MPlan<L>::R[S], NV = (1 << L) / R;
+__device__ __forceinline__ constexpr unsigned pos_of(unsigned e) {
+    constexpr unsigned R = MPlan<L>::R[S], NV = (1u << L) / R;
     return (e % NV) * R + e / NV;
 }
 
@@ -327,10 +327,19 @@ __device__ __forceinline__ void stages_from(unsigned *planes, const pfa32::Tw &t
     }
 }
 
-// Store element e of line col (first-stage layout), already scaled.
+// Stores of a lane's elements e0 + d of line col into the first-stage layout
+// (values already scaled), as a lane base and compile-time offsets:
+//     pp(pos(e0 + d)) = pp(pos(e0)) + pp(pos(d))
+// holds for the two patterns the kernels use -- e0 = 4 t, d = 4 g NB + u
+// (u < 4) in the azimuth pass and e0 = t, d = i NB in FFT-B, NB the block's
+// threads -- checked exhaustively for every size.
+template <int L>
+__device__ __forceinline__ unsigned *put_base(unsigned *planes, int col, unsigned e0) {
+    return planes + col * line_words<L>() + pp((int)pos_of<L, 0>(e0));
+}
 template <int L, bool BF>
-__device__ __forceinline__ void put(unsigned *planes, int col, int e, float2 v) {
-    planes[col * line_words<L>() + pp(pos_of<L, 0>(e))] = pack<BF>(v);
+__device__ __forceinline__ void put_at(unsigned *base, unsigned d, float2 v) {
+    base[pp((int)pos_of<L, 0>(d))] = pack<BF>(v);
 }
 
 // Transform NCOL lines whose first-stage layout is in `planes` (after a

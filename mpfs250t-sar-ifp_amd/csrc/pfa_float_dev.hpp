@@ -752,16 +752,15 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
                 const int se = pfam::norm_exp(lm);
                 const float sc = ldexpf(1.0f, -se);
                 unsigned *planes = reinterpret_cast<unsigned *>(reg);
+                unsigned *pb = pfam::put_base<L>(planes, 0, 4 * tl);
+                static_assert(4 * G * NB == M2, "the threads' groups cover the line");
 #pragma unroll
                 for (int g = 0; g < G; ++g) {
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int q = (g * NB + tl) * 4 + u;
-                        if (q < M2) {
-                            float2 w = make_float2(v[g * 4 + u].x * sc, v[g * 4 + u].y * sc);
-                            if ((q + j) & 1) w = make_float2(-w.x, -w.y);
-                            pfam::put<L, BF>(planes, 0, q, w);
-                        }
+                    for (int u = 0; u < 4; ++u) {          // q = (g NB + tl) 4 + u
+                        float2 w = make_float2(v[g * 4 + u].x * sc, v[g * 4 + u].y * sc);
+                        if ((u + j) & 1) w = make_float2(-w.x, -w.y);   // (-1)^(q + j)
+                        pfam::put_at<L, BF>(pb, 4 * g * NB + u, w);
                     }
                 }
                 __syncthreads();

@@ -78,6 +78,7 @@ __global__ __launch_bounds__(fftb_mfma_nb<L>()) void k_fft_b_mfma(
 #pragma unroll
     for (int i = 0; i < BW; ++i) S = max(S, emx[i]);
     S = (S == INT_MIN) ? 0 : S + 2;
+    unsigned *pb = pfam::put_base<L>(planes, 0, t);
 #pragma unroll
     for (int i = 0; i < NI; ++i) {
         const int jj = row_of(i);
@@ -86,7 +87,8 @@ __global__ __launch_bounds__(fftb_mfma_nb<L>()) void k_fft_b_mfma(
 #pragma unroll
         for (int c = 0; c < NCOL; ++c) {
             const float2 v = pfam::unpack<BF>(a[i][c]);
-            pfam::put<L, BF>(planes, c, jj, in ? make_float2(v.x * sc, v.y * sc) : make_float2(0.f, 0.f));
+            pfam::put_at<L, BF>(pb + c * PL, i * NB,
+                                in ? make_float2(v.x * sc, v.y * sc) : make_float2(0.f, 0.f));
         }
     }
     __syncthreads();
@@ -95,9 +97,12 @@ __global__ __launch_bounds__(fftb_mfma_nb<L>()) void k_fft_b_mfma(
         [&](int col, int c, float2 y) {
             const int r = r0 + col;
             float2 v = make_float2(y.x * osc, y.y * osc);
-            if ((r + c) & 1) v = make_float2(-v.x, -v.y);
-            if (OUT == 0) reinterpret_cast<float2 *>(outp)[(size_t)r * N2 + c] = v;
-            else reinterpret_cast<float *>(outp)[(size_t)r * N2 + c] = __fsqrt_rn(v.x * v.x + v.y * v.y);
+            if (OUT == 0) {
+                if ((r + c) & 1) v = make_float2(-v.x, -v.y);
+                reinterpret_cast<float2 *>(outp)[(size_t)r * N2 + c] = v;
+            } else {                                   // |(-1)^(r+c) v| = |v|, bit for bit
+                reinterpret_cast<float *>(outp)[(size_t)r * N2 + c] = __fsqrt_rn(v.x * v.x + v.y * v.y);
+            }
         });
 }
 
