@@ -175,11 +175,36 @@ __device__ __forceinline__ void pass(float2 *buf, const Tw &tw, Src *src, Sink &
             const int j = k % LS;
             if (LS > 1) {
                 constexpr int step = N / (LS * R);
+                if constexpr (R >= 8) {
+                    // W^(j r step) for r = 4 a + c from two lookups, w1 = W^(j step)
+                    // and w4 = W^(4 j step) (< N / 2), and at most four products deep:
+                    // w2 = w1 w1, w3 = w2 w1, W^(4a) = w4^a, W^r = W^(4a) w_c.
+                    float2 w1 = tw(j * step), w4 = tw(4 * j * step);
+                    if (INV) {
+                        w1.y = -w1.y;
+                        w4.y = -w4.y;
+                    }
+                    const float2 w2 = cmul(w1, w1), w3 = cmul(w2, w1);
+                    const float2 wc[4] = {make_float2(1.f, 0.f), w1, w2, w3};
+                    float2 wa = w4;
 #pragma unroll
-                for (int r = 1; r < R; ++r) {
-                    float2 w = tw(j * r * step);
-                    if (INV) w.y = -w.y;
-                    x[b][r] = cmul(x[b][r], w);
+                    for (int a = 0; a < R / 4; ++a) {
+                        if (a >= 2) wa = cmul(wa, w4);
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) {
+                            const int r = 4 * a + c;
+                            if (r == 0) continue;
+                            const float2 w = a == 0 ? wc[c] : (c == 0 ? wa : cmul(wa, wc[c]));
+                            x[b][r] = cmul(x[b][r], w);
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int r = 1; r < R; ++r) {
+                        float2 w = tw(j * r * step);
+                        if (INV) w.y = -w.y;
+                        x[b][r] = cmul(x[b][r], w);
+                    }
                 }
             }
             dft<R, INV>(x[b]);
