@@ -839,9 +839,12 @@ __global__ __launch_bounds__(fb2_nb<L>(), 4) void k_fft_b2_f32(
     auto sink_for = [&](int r) {
         return [&, r](int c, float2 v) {
             if (INV) v = make_float2(v.x * scale, v.y * scale);
-            if ((r + c) & 1) v = make_float2(-v.x, -v.y);
-            if (OUT == 0) reinterpret_cast<float2 *>(outp)[(size_t)r * N2 + c] = v;
-            else reinterpret_cast<float *>(outp)[(size_t)r * N2 + c] = __fsqrt_rn(v.x * v.x + v.y * v.y);
+            if (OUT == 0) {
+                if ((r + c) & 1) v = make_float2(-v.x, -v.y);
+                reinterpret_cast<float2 *>(outp)[(size_t)r * N2 + c] = v;
+            } else {                                   // |(-1)^(r+c) v| = |v|, bit for bit
+                reinterpret_cast<float *>(outp)[(size_t)r * N2 + c] = __fsqrt_rn(v.x * v.x + v.y * v.y);
+            }
         };
     };
     pfa32::fft_src<L, INV, FB_EMAX>(buf, tw, [&](int, int s2) { return c0[s2]; }, sink_for(r0));
