@@ -239,34 +239,49 @@ __global__ __launch_bounds__(az_nb<L>()) void k_af_az_adj(
             float2 H[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) H[u] = make_float2(0.f, 0.f);
-            for (int q = qa; q < qb; ++q) {
-                const int cw = code[q];
-                const int c = cw >> 8, ph = cw & 255;
-                if (c == 0) continue;
-                const float2 gq = reg[pfa32::padf(q)];
-                if (c == m) {                          // np.interp's x == x_last: value of the last sample
-                    if (ph) {
+            // Outputs in batches of 4: every batch's code, value and fraction loads
+            // are issued before any is used (a slot past qb reads as c = 0 and is
+            // skipped), and the sums still run over q in increasing order.
+            for (int q0 = qa; q0 < qb; q0 += 4) {
+                int cwb[4];
+                float2 gqb[4];
+                float frb[4];
 #pragma unroll
-                        for (int u = 0; u < 4; ++u)
-                            if (p0 + u == m - 1) H[u] = make_float2(H[u].x + gq.x, H[u].y + gq.y);
-                    }
-                    continue;
+                for (int i = 0; i < 4; ++i) {
+                    const int q = min(q0 + i, qb - 1);
+                    cwb[i] = q0 + i < qb ? code[q] : 0;
+                    gqb[i] = reg[pfa32::padf(q)];
+                    frb[i] = frq[q];
                 }
-                const int J = c - 1;
-                if (SINC && J >= 15 && J <= m - 17) {
-                    const float *col = tapl + scr(ph);
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const float h = col[(p0 + u - J + 15 + 4) * 256];   // zero rows outside the taps
-                        H[u] = make_float2(fmaf(h, gq.x, H[u].x), fmaf(h, gq.y, H[u].y));
+                for (int i = 0; i < 4; ++i) {
+                    const int c = cwb[i] >> 8, ph = cwb[i] & 255;
+                    if (c == 0) continue;
+                    const float2 gq = gqb[i];
+                    if (c == m) {                      // np.interp's x == x_last: value of the last sample
+                        if (ph) {
+#pragma unroll
+                            for (int u = 0; u < 4; ++u)
+                                if (p0 + u == m - 1) H[u] = make_float2(H[u].x + gq.x, H[u].y + gq.y);
+                        }
+                        continue;
                     }
-                } else {
-                    const float fr = frq[q];
+                    const int J = c - 1;
+                    if (SINC && J >= 15 && J <= m - 17) {
+                        const float *col = tapl + scr(ph);
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int pu = p0 + u;
-                        const float w = pu == J ? 1.0f - fr : (pu == J + 1 ? fr : 0.f);
-                        H[u] = make_float2(fmaf(w, gq.x, H[u].x), fmaf(w, gq.y, H[u].y));
+                        for (int u = 0; u < 4; ++u) {
+                            const float h = col[(p0 + u - J + 15 + 4) * 256];   // zero rows outside the taps
+                            H[u] = make_float2(fmaf(h, gq.x, H[u].x), fmaf(h, gq.y, H[u].y));
+                        }
+                    } else {
+                        const float fr = frb[i];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const int pu = p0 + u;
+                            const float w = pu == J ? 1.0f - fr : (pu == J + 1 ? fr : 0.f);
+                            H[u] = make_float2(fmaf(w, gq.x, H[u].x), fmaf(w, gq.y, H[u].y));
+                        }
                     }
                 }
             }
