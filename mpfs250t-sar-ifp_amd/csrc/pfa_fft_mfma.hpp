@@ -303,12 +303,22 @@ __device__ __forceinline__ void stage(unsigned *planes, const pfa32::Tw &tw, con
             const unsigned kq = j + 4 * q * LS;
             unsigned *dst = planes + col * PL + pp(((h % KB) * LS2 + kq) * R2 + r2);
             const unsigned T0 = kq * r2 * C, D = LS * r2 * C;
+            // W^(T0 + (16 mt + i) D) as products of W^T0, W^D and (R = 32) W^(16 D),
+            // at most four deep: a few FP32 ulp, far below the 16-bit rounding
+            // that follows.  16 D < N, since every exponent above is.
+            float2 w0 = tw((int)T0), wd = tw((int)D), w16 = make_float2(1.f, 0.f);
+            if (MT / 2 > 1) w16 = tw((int)(16 * D));
+            if (INV) {
+                w0.y = -w0.y;
+                wd.y = -wd.y;
+                w16.y = -w16.y;
+            }
 #pragma unroll
             for (int mt = 0; mt < MT / 2; ++mt) {
+                float2 wv = mt == 0 ? w0 : pfa32::cmul(w0, w16);
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    float2 wv = tw((int)(T0 + (16 * mt + i) * D));
-                    if (INV) wv.y = -wv.y;
+                    if (i) wv = pfa32::cmul(wv, wd);
                     const float2 y = pfa32::cmul(make_float2(acc[g][mt][i], acc[g][mt + MT / 2][i]), wv);
                     dst[pp((16 * mt + i) * LS * R2)] = pack<BF>(y);
                 }

@@ -160,6 +160,8 @@ def test_mfma_stage_epilogue(L):
         kq = j + 4 * q * ls
         p0 = ((h % kb) * ls2 + kq) * R2 + r2
         T0, D = kq * r2 * C, ls * r2 * C
+        # the lanes look up W^T0, W^D and (R = 32) W^(16 D): all inside the table
+        assert T0.max() < n and D.max() < n and (R // 16 == 1 or (16 * D).max() < n)
         obase = (k // ls) * ls * R + j
         for mt in range(R // 16):
             for i in range(4):
