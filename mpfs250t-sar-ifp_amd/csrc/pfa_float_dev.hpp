@@ -447,8 +447,6 @@ __global__ __launch_bounds__(RT, 2) void k_range_f32(
 }
 
 // ------------------------------------------------------------------ pass 2 (+ window + FFT A)
-// A is row-major [range bin j][cross r]: every workgroup of the azimuth
-// kernel writes whole 128-byte lines.
 // A, the azimuth pass's output [range row j][cross bin r], is stored in blocks
 // of 8 cross bins: element (j, r) at ((r / 8) nj + j) 8 + r % 8, with nj = the
 // row count rounded up to 16.  A line's FFT writes whole 64-byte chunks

@@ -8,9 +8,9 @@
 // Kernels
 //   k_azimuth_f32<.., FK>  azimuth pass + window + sign fold (FP32), then the
 //                          FFT along cross on the matrix cores -> A16 (16-bin blocks), expA[j]
-//   k_fft_b_mfma           FFT along range for 4 adjacent cross bins per
-//                          workgroup (16-byte loads of A16 rows = the corner-turn),
-//                          output sign fold and detection in the store
+//   k_fft_b_mfma           FFT along range for FBM_NCOL = 2 adjacent cross bins per
+//                          workgroup (8-byte loads of A16 rows = the corner-turn),
+//                          output sign fold (complex output) or detection in the store
 #include "pfa_float_dev.hpp"
 
 #include <climits>
@@ -43,11 +43,12 @@ __global__ __launch_bounds__(fftb_mfma_nb<L>()) void k_fft_b_mfma(
     const pfa32::Tw tw = pfa32::load_tw<N2>(twl, twg);
     const int t = threadIdx.x;
     const int r0 = xcd_map(blockIdx.x, m2 / NCOL) * NCOL;
-    // the corner-turn: 4 adjacent cross bins of every range row.  A16 is in
+    // the corner-turn: NCOL adjacent cross bins of every range row.  A16 is in
     // 16-bin blocks, so consecutive rows of a block are 64 bytes apart and
-    // consecutive lanes take consecutive rows: a wave's 16-byte loads fall in
-    // 32 cache lines.  All loads are issued first (clamped rows); the row
-    // exponents loaded with them also give the block's common exponent.
+    // consecutive lanes take consecutive rows: a wave's 8-byte loads (NCOL = 2)
+    // span 64 rows x 64 bytes, 32 cache lines with two rows' runs in each.  All
+    // loads are issued first (clamped rows); the row exponents loaded with them
+    // also give the block's common exponent.
     constexpr int NI = N2 / NB;
     static_assert(N2 % NB == 0, "row assignment covers the line");
     const int nj = a_rows(n);
