@@ -51,7 +51,8 @@ def _ref64(m, n, seed, sgn, interp):
 
 CASES = [(1024, 1024, -1, "sinc32"),      # 1024 x 1024
          (300, 4100, -1, "linear"),       # 512 cross x 8192 range
-         (2100, 200, 1, "linear")]        # 4096 cross x 256 range, inverse transform
+         (2100, 200, 1, "linear"),        # 4096 cross x 256 range, inverse transform
+         (600, 500, 1, "linear")]         # 1024 cross x 512 range, inverse, radix-32 stages
 
 
 @pytest.mark.parametrize("m,n,sgn,interp", CASES)
