@@ -41,6 +41,18 @@ __device__ __forceinline__ int pad32(int i) { return i + (i >> 5); }
 // registers, and the local bit window moves through LDS between them.  Every
 // stage ends with the line-wide max that sets the shift (0/1/2) the reference
 // applies before the next stage.
+// Wave-wide max on DPP row operations: quad swaps and row rotations leave each
+// 16-lane row's max in all its lanes, two row broadcasts carry it to lane 63.
+__device__ __forceinline__ int wave_max_dpp(int v) {
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x124, 0xF, 0xF, false));  // row_ror:4
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x128, 0xF, 0xF, false));  // row_ror:8
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xA, 0xF, false));  // row_bcast:15
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x143, 0xC, 0xF, false));  // row_bcast:31
+    return v;                                          // the wave's max in lane 63
+}
+
 template <int L>
 struct Fft {
     static constexpr int N = 1 << L;
@@ -130,9 +142,8 @@ struct Fft {
                                          max(abs(ar - tr), abs(ai - ti))));
                 }
             }
-#pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) lmax = max(lmax, __shfl_xor(lmax, off, 64));
-            if ((t & 63) == 0) red[(S & 1) * NW + (t >> 6)] = lmax;
+            lmax = wave_max_dpp(lmax);
+            if ((t & 63) == 63) red[(S & 1) * NW + (t >> 6)] = lmax;
             __syncthreads();
             int mx = red[(S & 1) * NW];
 #pragma unroll
