@@ -333,11 +333,16 @@ __global__ __launch_bounds__(256) void k_range_resample(
     const int32_t *__restrict__ order, const float *__restrict__ krp,
     const int16_t *__restrict__ sinc, int flags, short2 *__restrict__ g1t, int ldp) {
     __shared__ short2 tile[QT][RT + 1];
+    __shared__ uint4 stl[256 * 4];                 // the 256 x 32 Q15 tap table, 16 KB
     const int p0 = blockIdx.x * RT, q0 = blockIdx.y * QT;
     const int t = threadIdx.x;
     const int pl = t >> 2, ch = t & 3;
     const int p = p0 + pl;
     const bool use_sinc = flags & PFA_FIXED_RANGE_SINC;
+    if (use_sinc)
+        for (int i = t; i < 256 * 4; i += 256) stl[i] = reinterpret_cast<const uint4 *>(sinc)[i];
+    __syncthreads();
+    const int16_t *sincl = reinterpret_cast<const int16_t *>(stl);
     const bool f32 = !(flags & PFA_FIXED_LERP_EXACT);
     if (p < m) {
         const int i = order[p];
@@ -375,7 +380,7 @@ __global__ __launch_bounds__(256) void k_range_resample(
                 const int idx = asc ? k : n - 2 - k;
                 const int wq = q15_weight(asc ? frac : 1.0 - frac);
                 if (use_sinc && idx >= 15 && idx <= n - 17)
-                    v = sinc_q15w(row, idx, sinc + (wq >> 7) * 32);
+                    v = sinc_q15w(row, idx, sincl + (wq >> 7) * 32);
                 else
                     v = lerp_q15(row, idx, wq, f32 && !use_sinc);
             }
