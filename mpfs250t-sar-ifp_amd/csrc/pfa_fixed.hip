@@ -24,6 +24,7 @@
 #include <cstring>
 #include <vector>
 
+#include "pfa_div.hpp"
 #include "pfa_internal.hpp"
 
 namespace {
@@ -376,7 +377,7 @@ __global__ __launch_bounds__(256) void k_range_resample(
             short2 v = make_short2(0, 0);
             if (k >= 0 && k <= n - 2) {
                 const double x0 = xa(k), x1 = xa(k + 1);
-                const double frac = (Q - x0) / (x1 - x0);
+                const double frac = div_bracket(Q - x0, x1 - x0);
                 const int idx = asc ? k : n - 2 - k;
                 const int wq = q15_weight(asc ? frac : 1.0 - frac);
                 if (use_sinc && idx >= 15 && idx <= n - 17)
@@ -474,7 +475,7 @@ __global__ __launch_bounds__(Fft<L>::NB) void k_azimuth_fft(
         unsigned code = 0;                                   // kind 0: zero output
         if (k >= 0 && k <= m - 2) {
             const double x0 = sa(k), x1 = sa(k + 1);
-            const double frac = ((double)Q - x0) / (x1 - x0);
+            const double frac = div_bracket((double)Q - x0, x1 - x0);
             const int idx = asc ? k : m - 2 - k;
             const int wq = q15_weight(asc ? frac : 1.0 - frac);
             code = gather_code(use_sinc && idx >= 15 && idx <= m - 17 ? 1 : 2, idx, wq);

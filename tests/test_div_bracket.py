@@ -1,8 +1,8 @@
-"""The bracket-fraction division of the FP32 resample kernels (div_bracket in
-csrc/pfa_float_dev.hpp): reciprocal, two Newton steps, the quotient and one
+"""The bracket-fraction division of the resample kernels (div_bracket in
+csrc/pfa_div.hpp, float and fixed passes): reciprocal, two Newton steps, the quotient and one
 residual correction, every FMA and product rounded once to float64.
 
-For 0 <= n < d of the magnitudes positions take, the result must be the
+For 0 <= n <= d of the magnitudes positions take, the result must be the
 correctly rounded n / d -- what np.interp's division gives -- whatever the last
 bits of the first reciprocal are (the hardware reciprocal is an
 approximation; here it is perturbed by up to 2^-14 relative).  Every
@@ -33,8 +33,8 @@ def _cases(rng, count):
         d = float(10.0 ** rng.uniform(-6, 3))
         out.append((float(d * rng.uniform(0.0, 1.0)), d))
     for d in (1.0, 0.5, 3.0, 1e-6, 1e3, 0.1):
-        out += [(0.0, d), (float(np.nextafter(d, 0.0)), d), (d / 3.0, d), (d * 0.5, d)]
-    return [(n, d) for n, d in out if 0.0 <= n < d]
+        out += [(0.0, d), (float(np.nextafter(d, 0.0)), d), (d / 3.0, d), (d * 0.5, d), (d, d)]
+    return [(n, d) for n, d in out if 0.0 <= n <= d]
 
 
 def test_div_bracket_is_correctly_rounded():
