@@ -197,7 +197,7 @@ __global__ __launch_bounds__(az_nb<L>()) void k_af_az_adj(
             if (c == m) {
                 ph = (Q == xl) ? 1 : 0;
             } else if (c > 0) {
-                const double f = __ddiv_rn(__dsub_rn(Q, xl), __dsub_rn(xh, xl));
+                const double f = div_bracket(__dsub_rn(Q, xl), __dsub_rn(xh, xl));
                 ph = (int)fmin(rint(f * 32768.0), 32767.0) >> 7;
                 fr = (float)f;
             }
