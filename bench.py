@@ -301,6 +301,8 @@ def autofocus_c5(interp, steps_af, torch):
 
     inj = detrend(sc.phase_error)
     res = detrend(phi_hat.cpu().numpy().astype(np.float64) + sc.phase_error)
+    w_inj = wrapped_residual(sc.phase_error)
+    w_res = wrapped_residual(phi_hat.cpu().numpy().astype(np.float64) + sc.phase_error)
     af.close()
     return {"workload": f"C5: {m} pulses x {n} samples -> 4096^2, {interp}, FP32, "
                         "quadratic 3 rad + random walk 0.05 rad/pulse phase error",
@@ -317,7 +319,30 @@ def autofocus_c5(interp, steps_af, torch):
                                                  round(float(res.std()), 4)],
             "phase_error_rms_rad_lowpass64_before_after": [
                 round(float(lowpass(inj)[64:-64].std()), 4),
-                round(float(lowpass(res)[64:-64].std()), 4)]}
+                round(float(lowpass(res)[64:-64].std()), 4)],
+            # modulo 2 pi per pulse (S cannot tell phi from phi + 2 pi k) and
+            # modulo the constant + linear phase (an image shift)
+            "phase_residual_wrapped_rms_rad_before_after": [round(w_inj[0], 4),
+                                                            round(w_res[0], 4)],
+            "phase_coherence_before_after": [round(w_inj[1], 4), round(w_res[1], 4)]}
+
+
+def wrapped_residual(res, pad=64):
+    """A per-pulse phase residual modulo 2 pi and modulo its best constant +
+    linear phase: the ramp b maximising |sum_i exp(j (res_i - b i))| (a
+    zero-padded FFT, then a parabolic refinement), the coherence
+    |sum_i exp(j (res_i - b i))| / m, and the RMS of the wrapped remainder."""
+    m = res.size
+    z = np.exp(1j * res)
+    spec = np.fft.fft(z, pad * m)
+    k = int(np.argmax(np.abs(spec)))
+    a0, a1, a2 = (np.abs(spec[(k + d) % (pad * m)]) for d in (-1, 0, 1))
+    den = a0 - 2 * a1 + a2
+    kk = k + (0.5 * (a0 - a2) / den if den != 0 else 0.0)
+    b = 2 * np.pi * kk / (pad * m)
+    c = np.sum(z * np.exp(-1j * b * np.arange(m)))
+    r = np.angle(z * np.exp(-1j * b * np.arange(m)) * np.exp(-1j * np.angle(c)))
+    return float(np.sqrt(np.mean(r * r))), float(np.abs(c) / m)
 
 
 def cpu_baseline(spec, interp, geom, budget_rows=800):

@@ -21,7 +21,8 @@
  *     always host pointers and are copied by the library.
  *   - Every function returns PFA_OK (0) or a negative pfa_status; the message
  *     is available from pfa_last_error(ctx).  Nothing aborts or throws across
- *     the ABI.  One context per host thread.
+ *     the ABI: a host allocation or thread failure inside a call returns
+ *     PFA_ERR_NOMEM.  One context per host thread.
  */
 #ifndef MI355X_PFA_H
 #define MI355X_PFA_H
@@ -108,6 +109,7 @@ int pfa_window_fixed(pfa_ctx *ctx, const int16_t *g2_iq, int rows, int cols,
 
 /* detect_fixed (silicon_emulator.py:208-213): floor(sqrt(I^2+Q^2)) -> uint16. */
 int pfa_detect_fixed(pfa_ctx *ctx, const int16_t *z_iq, int64_t count, uint16_t *out);
+int pfa_detect_fixed_dev(pfa_ctx *ctx, const int16_t *z_iq, int64_t count, uint16_t *out);
 
 /* Scene tables of silicon_emulator.py:128-140 (the board's DDR roles,
  * serialize_inputs.py:203-230): f0, df, pr, tan_s float32[m]; inv_order
@@ -177,6 +179,35 @@ int pfa_plan_float(pfa_ctx *ctx, pfa_plan **plan, int m, int n, int sgn,
                    const double *f0, const double *df,
                    const double *ham_c, const double *ham_r,
                    const int16_t *sinc_tab, int flags);
+/* The O(m + n) scene geometry above from the reference's own arguments, on
+ * the host (replaces form_image_pfa.resample_kspace's geometry,
+ * form_image_pfa.py:178-193, 205-209, 219-221): the rotated frame from the
+ * mean look direction (ndarray.mean's blocked pairwise sum, math.hypot
+ * correctly rounded), pr, tan(phi), the pulse order (ties in tan(phi) keep
+ * pulse index order), KR over the pulses' common kr overlap (np.linspace),
+ * KC, np.hamming tapers, and geo = {dc, dr, dhat_x, dhat_y, chat_x, chat_y}
+ * of the unpadded grid.  freq is the full float64 [m][n] array (its O(m n)
+ * kr extrema are scanned here), or NULL with compact f0, df [m] (freq[i][k] =
+ * f0[i] + k df[i], each operation rounded).  ham_c[m] and ham_r[n] may both be
+ * NULL.  Needs no device. */
+int pfa_float_geometry(int m, int n, const double *freq, const double *f0, const double *df,
+                       const double *ax, const double *ay, double *pr, int32_t *order,
+                       double *tan_s, double *kr_grid, double *kc_grid, double *ham_c,
+                       double *ham_r, double geo[6]);
+/* pfa_plan_float from those arguments (form_image_pfa.pfa's own: freq or
+ * f0/df, ax, ay, sgn); window != 0 applies the np.hamming tapers.  geo
+ * receives pfa's geo: the spacing scaled to the padded grid (focus,
+ * form_image_pfa.py:237-238). */
+int pfa_plan_float_pfa(pfa_ctx *ctx, pfa_plan **plan, int m, int n, int sgn,
+                       const double *freq, const double *f0, const double *df,
+                       const double *ax, const double *ay, int window,
+                       const int16_t *sinc_tab, int flags, double geo[6]);
+/* form_image_pfa.pfa(sig, freq, ax, ay, sgn) (form_image_pfa.py:243-246) in
+ * one synchronous call on host buffers: the windowed np.interp focus (flags
+ * may select the FFT form only), out [m2][n2] as out_kind, geo as above. */
+int pfa_pfa_f32(pfa_ctx *ctx, const float *sig_c64, int m, int n, const double *freq,
+                const double *ax, const double *ay, int sgn, int flags, void *out,
+                int out_kind, double geo[6]);
 /* pfa (form_image_pfa.py:243-246) / NumpyBackend.focus (accel.py:39-44):
  * sig complex64 [m][n] -> out [m2][n2] complex64 (PFA_OUT_C64) or |img|
  * float32 (PFA_OUT_ABS_F32).  freq: full float64 [m][n] or NULL (compact). */
@@ -247,7 +278,8 @@ int pfa_stretch_u8_dev(pfa_ctx *ctx, const float *x, int64_t count, float d, uin
  * padded cross size m2 <= 4096.  All pointers are device pointers; work is
  * enqueued on the context stream.
  *   pfa_af_checkpoint  range pass of sig once (the phase commutes with it); its
- *                      corner-turned output stays in the plan's workspace
+ *                      corner-turned output stays in a buffer of this plan's
+ *                      own (plans on one context never share it)
  *   pfa_af_forward     img[m2][n2] complex64 of the corrected history and
  *                      stats[3] = {sum |I|^2, sum |I|^4, S} (double)
  *   pfa_af_backward    grad[m] = dS/dphi from img and stats of that forward,
@@ -258,6 +290,10 @@ int pfa_af_forward(pfa_plan *plan, const float *phi, float *img_c64, double *sta
 int pfa_af_backward(pfa_plan *plan, const float *phi, const float *img_c64,
                     const double *stats, float *grad);
 
+/* Releases a plan.  Plans are normally destroyed before their context; a
+ * plan still alive when pfa_destroy runs loses its device tables there, every
+ * later call on it returns PFA_ERR_STATE, and pfa_plan_destroy then frees
+ * only the plan object. */
 void pfa_plan_destroy(pfa_plan *plan);
 
 #ifdef __cplusplus

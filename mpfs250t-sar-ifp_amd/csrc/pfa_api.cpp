@@ -8,14 +8,37 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <system_error>
 #include <vector>
 
 #include "pfa_internal.hpp"
 
 // ------------------------------------------------------------------ errors / workspace
-int pfa_fail(pfa_ctx *ctx, int code, const std::string &msg) {
-    if (ctx) ctx->err = msg;
+int pfa_fail(pfa_ctx *ctx, int code, const char *msg) noexcept {
+    if (ctx) {
+        try {
+            ctx->err = msg;
+        } catch (...) {                   // no memory for the message: keep the code
+            ctx->err.clear();
+        }
+    }
     return code;
+}
+
+int pfa_fail(pfa_ctx *ctx, int code, const std::string &msg) { return pfa_fail(ctx, code, msg.c_str()); }
+
+int pfa_exception(pfa_ctx *ctx) noexcept {
+    try {
+        throw;
+    } catch (const std::bad_alloc &) {
+        return pfa_fail(ctx, PFA_ERR_NOMEM, "host memory exhausted");
+    } catch (const std::system_error &e) {
+        return pfa_fail(ctx, PFA_ERR_NOMEM, e.what());
+    } catch (const std::exception &e) {
+        return pfa_fail(ctx, PFA_ERR_STATE, e.what());
+    } catch (...) {
+        return pfa_fail(ctx, PFA_ERR_STATE, "unexpected exception");
+    }
 }
 
 int pfa_hip_fail(pfa_ctx *ctx, hipError_t e, const char *what) {
@@ -71,6 +94,7 @@ void pfa_timer_end(pfa_ctx *ctx, int kind) {
 extern "C" int pfa_abi_version(void) { return PFA_ABI_VERSION; }
 
 extern "C" int pfa_create(pfa_ctx **out, int device) {
+    PFA_TRY
     if (!out) return PFA_ERR_ARG;
     *out = nullptr;
     int ndev = 0;
@@ -96,12 +120,31 @@ extern "C" int pfa_create(pfa_ctx **out, int device) {
     }
     *out = ctx;
     return PFA_OK;
+    PFA_CATCH(nullptr)
+}
+
+static int plan_register(pfa_plan *p, pfa_plan **out);
+
+// Releases a plan's device tables (the plan object itself stays until
+// pfa_plan_destroy).
+static void plan_release(pfa_plan *p) noexcept {
+    for (void *d : p->owned) hipFree(d);
+    p->owned.clear();
+    p->af_g1t = nullptr;
+    p->af_ready = false;
 }
 
 extern "C" void pfa_destroy(pfa_ctx *ctx) {
     if (!ctx) return;
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
+    // plans still alive are orphaned: their device tables go with the context,
+    // and every later call on them returns PFA_ERR_STATE until pfa_plan_destroy
+    for (pfa_plan *p : ctx->plans) {
+        plan_release(p);
+        p->ctx = nullptr;
+    }
+    ctx->plans.clear();
     pfa_hostio_destroy(ctx);
     for (auto &kv : ctx->ws)
         if (kv.second.ptr) hipFree(kv.second.ptr);
@@ -121,24 +164,31 @@ extern "C" const char *pfa_last_error(const pfa_ctx *ctx) {
 }
 
 extern "C" int pfa_set_stream(pfa_ctx *ctx, void *s) {
+    PFA_TRY
     if (!ctx) return PFA_ERR_ARG;
     ctx->stream = s ? (hipStream_t)s : ctx->own_stream;
     return PFA_OK;
+    PFA_CATCH(ctx)
 }
 
 extern "C" int pfa_synchronize(pfa_ctx *ctx) {
+    PFA_TRY
     if (!ctx) return PFA_ERR_ARG;
     PFA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return PFA_OK;
+    PFA_CATCH(ctx)
 }
 
 extern "C" int pfa_timer_enable(pfa_ctx *ctx, int on) {
+    PFA_TRY
     if (!ctx) return PFA_ERR_ARG;
     ctx->timing = on != 0;
     return PFA_OK;
+    PFA_CATCH(ctx)
 }
 
 extern "C" int pfa_timer_reset(pfa_ctx *ctx) {
+    PFA_TRY
     if (!ctx) return PFA_ERR_ARG;
     PFA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     for (auto &mk : ctx->marks) {
@@ -147,9 +197,11 @@ extern "C" int pfa_timer_reset(pfa_ctx *ctx) {
     }
     ctx->marks.clear();
     return PFA_OK;
+    PFA_CATCH(ctx)
 }
 
 extern "C" int pfa_timer_read(pfa_ctx *ctx, int kind, float *ms_total, int *launches) {
+    PFA_TRY
     if (!ctx || !ms_total || !launches || kind < 0 || kind >= PFA_TIMER_KINDS) return PFA_ERR_ARG;
     PFA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     float tot = 0.f;
@@ -164,18 +216,22 @@ extern "C" int pfa_timer_read(pfa_ctx *ctx, int kind, float *ms_total, int *laun
     *ms_total = tot;
     *launches = cnt;
     return PFA_OK;
+    PFA_CATCH(ctx)
 }
 
 extern "C" int pfa_fallback_counts(pfa_ctx *ctx, uint32_t out[2], int reset) {
+    PFA_TRY
     if (!ctx || !out) return PFA_ERR_ARG;
     PFA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     PFA_HIP(ctx, hipMemcpy(out, ctx->d_counters + 32, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost));
     if (reset) PFA_HIP(ctx, hipMemset(ctx->d_counters + 32, 0, 2 * sizeof(uint32_t)));
     return PFA_OK;
+    PFA_CATCH(ctx)
 }
 
 // ------------------------------------------------------------------ tables
 extern "C" int pfa_twiddle_codes(int n, int bits, int32_t *out) {
+    PFA_TRY
     if (n < 2 || (n & (n - 1)) || bits < 2 || bits > 30 || !out) return PFA_ERR_ARG;
     const double scale = std::ldexp(1.0, bits - 1);
     const double full = scale - 1.0;
@@ -188,6 +244,7 @@ extern "C" int pfa_twiddle_codes(int n, int bits, int32_t *out) {
         out[2 * k + 1] = (int32_t)std::min(std::max(im, -full - 1.0), full);
     }
     return PFA_OK;
+    PFA_CATCH(nullptr)
 }
 
 // ------------------------------------------------------------------ helpers
@@ -215,12 +272,15 @@ static T *upload(pfa_plan *p, const T *h, size_t count, int *st) {
 // ------------------------------------------------------------------ fixed: stage functions
 extern "C" int pfa_quantize_iq16_dev(pfa_ctx *ctx, const float *sig, int64_t count, int16_t *out,
                                      int *exp_out) {
+    PFA_TRY
     if (!ctx || (count && (!sig || !out)) || !exp_out || count < 0) return pfa_fail(ctx, PFA_ERR_ARG, "quantize: bad argument");
     return launch_quantize(ctx, sig, count * 2, out, exp_out);
+    PFA_CATCH(ctx)
 }
 
 extern "C" int pfa_quantize_iq16(pfa_ctx *ctx, const float *sig, int64_t count, int16_t *out,
                                  int *exp_out) {
+    PFA_TRY
     if (!ctx || (count && (!sig || !out)) || !exp_out || count < 0) return pfa_fail(ctx, PFA_ERR_ARG, "quantize: bad argument");
     int st = PFA_OK;
     const size_t in_b = (size_t)count * 8, out_b = (size_t)count * 4;
@@ -232,6 +292,7 @@ extern "C" int pfa_quantize_iq16(pfa_ctx *ctx, const float *sig, int64_t count, 
     CK(launch_quantize(ctx, d_in, count * 2, d_out, exp_out));
     CK(pfa_io_d2h(ctx, out, d_out, out_b));
     return PFA_OK;
+    PFA_CATCH(ctx)
 }
 
 static int fft_rows_dev(pfa_ctx *ctx, const int16_t *x, int rows, int n, int16_t *y, int32_t *e,
@@ -250,7 +311,9 @@ static int fft_rows_dev(pfa_ctx *ctx, const int16_t *x, int rows, int n, int16_t
 
 extern "C" int pfa_fft_pass_bfp_dev(pfa_ctx *ctx, const int16_t *x, int rows, int n, int16_t *y,
                                     int32_t *e) {
+    PFA_TRY
     return fft_rows_dev(ctx, x, rows, n, y, e, true);
+    PFA_CATCH(ctx)
 }
 
 static int fft_rows_host(pfa_ctx *ctx, const int16_t *x, int rows, int n, int16_t *y, int32_t *e,
@@ -276,16 +339,21 @@ static int fft_rows_host(pfa_ctx *ctx, const int16_t *x, int rows, int n, int16_
 
 extern "C" int pfa_fft_pass_bfp(pfa_ctx *ctx, const int16_t *x, int rows, int n, int16_t *y,
                                 int32_t *e) {
+    PFA_TRY
     return fft_rows_host(ctx, x, rows, n, y, e, true);
+    PFA_CATCH(ctx)
 }
 
 extern "C" int pfa_fft_bfp_perrow(pfa_ctx *ctx, const int16_t *x, int rows, int n, int16_t *y,
                                   int32_t *e) {
+    PFA_TRY
     return fft_rows_host(ctx, x, rows, n, y, e, false);
+    PFA_CATCH(ctx)
 }
 
 extern "C" int pfa_window_fixed(pfa_ctx *ctx, const int16_t *g2, int rows, int cols,
                                 const int32_t *hamr, const int32_t *hamc, int16_t *out) {
+    PFA_TRY
     if (!ctx) return PFA_ERR_ARG;
     if (rows < 0 || cols < 0 || ((rows && cols) && (!g2 || !hamr || !hamc || !out)))
         return pfa_fail(ctx, PFA_ERR_ARG, "window: bad argument");
@@ -308,9 +376,20 @@ extern "C" int pfa_window_fixed(pfa_ctx *ctx, const int16_t *g2, int rows, int c
     CK(launch_window_fixed(ctx, dx, rows, cols, dh, dh + rows, dy));
     CK(pfa_io_d2h(ctx, out, dy, b));
     return PFA_OK;
+    PFA_CATCH(ctx)
+}
+
+extern "C" int pfa_detect_fixed_dev(pfa_ctx *ctx, const int16_t *z, int64_t count, uint16_t *out) {
+    PFA_TRY
+    if (!ctx) return PFA_ERR_ARG;
+    if (count < 0 || (count && (!z || !out))) return pfa_fail(ctx, PFA_ERR_ARG, "detect: bad argument");
+    if (!count) return PFA_OK;
+    return launch_detect_fixed(ctx, z, count, out);
+    PFA_CATCH(ctx)
 }
 
 extern "C" int pfa_detect_fixed(pfa_ctx *ctx, const int16_t *z, int64_t count, uint16_t *out) {
+    PFA_TRY
     if (!ctx) return PFA_ERR_ARG;
     if (count < 0 || (count && (!z || !out))) return pfa_fail(ctx, PFA_ERR_ARG, "detect: bad argument");
     if (!count) return PFA_OK;
@@ -323,6 +402,7 @@ extern "C" int pfa_detect_fixed(pfa_ctx *ctx, const int16_t *z, int64_t count, u
     CK(launch_detect_fixed(ctx, dz, count, (uint16_t *)dy));
     CK(pfa_io_d2h(ctx, out, dy, (size_t)count * 2));
     return PFA_OK;
+    PFA_CATCH(ctx)
 }
 
 // ------------------------------------------------------------------ fixed: plan + pipeline
@@ -331,6 +411,7 @@ extern "C" int pfa_plan_fixed(pfa_ctx *ctx, pfa_plan **out, int m, int n, int gr
                               const float *tan_s, const int32_t *inv_order, const float *krp,
                               const float *kcp, const int32_t *hamr, const int32_t *hamc,
                               const int16_t *sinc_tab, int flags) {
+    PFA_TRY
     if (!ctx || !out) return PFA_ERR_ARG;
     *out = nullptr;
     if (!f0 || !df || !pr || !tan_s || !inv_order || !krp || !kcp || !hamr || !hamc)
@@ -385,19 +466,43 @@ extern "C" int pfa_plan_fixed(pfa_ctx *ctx, pfa_plan **out, int m, int n, int gr
               (p->kcp = upload(p, kcp, grid, &st)) && (p->hamr = upload(p, hamr, grid, &st)) &&
               (p->hamc = upload(p, hamc, grid, &st));
     if (ok && sinc_tab) ok = (p->sinc = upload(p, sinc_tab, 256 * 32, &st)) != nullptr;
+    if (ok) {                                     // the FFT passes' twiddles, set up with the plan
+        std::vector<int32_t> codes(grid);
+        pfa_twiddle_codes(grid, 16, codes.data());
+        std::vector<int16_t> tw(grid);
+        for (int k = 0; k < grid; ++k) tw[k] = (int16_t)codes[k];
+        ok = (p->tw16 = upload(p, tw.data(), grid, &st)) != nullptr;
+    }
     if (!ok) {
         pfa_plan_destroy(p);
         return st;
     }
-    *out = p;
-    return PFA_OK;
+    return plan_register(p, out);
+    PFA_CATCH(ctx)
 }
 
 extern "C" void pfa_plan_destroy(pfa_plan *p) {
     if (!p) return;
-    hipStreamSynchronize(p->ctx->stream);
-    for (void *d : p->owned) hipFree(d);
+    if (p->ctx) {                                 // null: orphaned by pfa_destroy
+        hipStreamSynchronize(p->ctx->stream);
+        plan_release(p);
+        p->ctx->plans.erase(p);
+    }
     delete p;
+}
+
+// The plan is registered with its context once built (pfa_destroy orphans the
+// plans it still holds).
+static int plan_register(pfa_plan *p, pfa_plan **out) {
+    try {
+        p->ctx->plans.insert(p);
+    } catch (...) {
+        pfa_ctx *ctx = p->ctx;
+        pfa_plan_destroy(p);
+        return pfa_fail(ctx, PFA_ERR_NOMEM, "host memory exhausted");
+    }
+    *out = p;
+    return PFA_OK;
 }
 
 static int ldp_of(int m) { return (m + 63) / 64 * 64; }
@@ -415,7 +520,8 @@ static int resample_fixed_dev(pfa_plan *p, const void *sig, void *g2) {
 }
 
 extern "C" int pfa_resample_fixed(pfa_plan *p, const int16_t *sig, int16_t *g2) {
-    if (!p || p->kind != PLAN_FIXED) return PFA_ERR_STATE;
+    PFA_TRY
+    if (!p || !p->ctx || p->kind != PLAN_FIXED) return PFA_ERR_STATE;
     pfa_ctx *ctx = p->ctx;
     if (!sig || !g2) return pfa_fail(ctx, PFA_ERR_ARG, "resample: null pointer");
     int st = PFA_OK;
@@ -428,11 +534,13 @@ extern "C" int pfa_resample_fixed(pfa_plan *p, const int16_t *sig, int16_t *g2) 
     CK(resample_fixed_dev(p, ds, dg));
     CK(pfa_io_d2h(ctx, g2, dg, out_b));
     return PFA_OK;
+    PFA_CATCH((p ? p->ctx : nullptr))
 }
 
 extern "C" int pfa_form_image_fixed_dev(pfa_plan *p, const int16_t *sig, uint16_t *out,
                                         int32_t *exp_a, int32_t *exp_b) {
-    if (!p || p->kind != PLAN_FIXED) return PFA_ERR_STATE;
+    PFA_TRY
+    if (!p || !p->ctx || p->kind != PLAN_FIXED) return PFA_ERR_STATE;
     pfa_ctx *ctx = p->ctx;
     if (!sig || !out) return pfa_fail(ctx, PFA_ERR_ARG, "form_image: null pointer");
     int st = PFA_OK;
@@ -453,11 +561,13 @@ extern "C" int pfa_form_image_fixed_dev(pfa_plan *p, const int16_t *sig, uint16_
     CK(launch_fixed_fft_b(p, ya, ea, emax, yb, eb, emax + 1));
     CK(launch_fixed_detect_t(p, yb, eb, emax + 1, out));
     return PFA_OK;
+    PFA_CATCH((p ? p->ctx : nullptr))
 }
 
 extern "C" int pfa_form_image_fixed(pfa_plan *p, const int16_t *sig, uint16_t *out,
                                     int32_t *exp_a, int32_t *exp_b) {
-    if (!p || p->kind != PLAN_FIXED) return PFA_ERR_STATE;
+    PFA_TRY
+    if (!p || !p->ctx || p->kind != PLAN_FIXED) return PFA_ERR_STATE;
     pfa_ctx *ctx = p->ctx;
     if (!sig || !out) return pfa_fail(ctx, PFA_ERR_ARG, "form_image: null pointer");
     int st = PFA_OK;
@@ -475,6 +585,7 @@ extern "C" int pfa_form_image_fixed(pfa_plan *p, const int16_t *sig, uint16_t *o
     if (exp_b) PFA_HIP(ctx, hipMemcpyAsync(exp_b, de + G, (size_t)G * 4, hipMemcpyDeviceToHost, ctx->stream));
     CK(pfa_io_d2h(ctx, out, dout, out_b));
     return PFA_OK;
+    PFA_CATCH((p ? p->ctx : nullptr))
 }
 
 // ------------------------------------------------------------------ float: plan + pipeline
@@ -491,6 +602,7 @@ extern "C" int pfa_plan_float(pfa_ctx *ctx, pfa_plan **out, int m, int n, int sg
                               const double *kr_grid, const double *kc_grid, const double *f0,
                               const double *df, const double *ham_c, const double *ham_r,
                               const int16_t *sinc_tab, int flags) {
+    PFA_TRY
     if (!ctx || !out) return PFA_ERR_ARG;
     *out = nullptr;
     if (!pr || !order || !tan_s || !kr_grid || !kc_grid)
@@ -569,8 +681,8 @@ extern "C" int pfa_plan_float(pfa_ctx *ctx, pfa_plan **out, int m, int n, int sg
     }
     if (linspace_params(kr_grid, n, p->kr_lin_v)) p->kr_lin = p->kr_lin_v;
     if (linspace_params(kc_grid, m, p->kc_lin_v)) p->kc_lin = p->kc_lin_v;
-    *out = p;
-    return PFA_OK;
+    return plan_register(p, out);
+    PFA_CATCH(ctx)
 }
 
 static int ldp32(int m) { return (m + 31) / 32 * 32; }
@@ -593,7 +705,7 @@ static bool linspace_params(const double *g, int cnt, double out[3]) {
 }
 
 static int float_check(pfa_plan *p, const void *sig, const double *freq, const void *out, int out_kind) {
-    if (!p || p->kind != PLAN_FLOAT) return PFA_ERR_STATE;
+    if (!p || !p->ctx || p->kind != PLAN_FLOAT) return PFA_ERR_STATE;
     if (!sig || !out) return pfa_fail(p->ctx, PFA_ERR_ARG, "float: null pointer");
     if (out_kind != PFA_OUT_C64 && out_kind != PFA_OUT_ABS_F32) return pfa_fail(p->ctx, PFA_ERR_ARG, "float: bad out_kind");
     if (!freq && !p->f0d) return pfa_fail(p->ctx, PFA_ERR_ARG, "float: plan has no compact frequencies; pass freq");
@@ -602,6 +714,7 @@ static int float_check(pfa_plan *p, const void *sig, const double *freq, const v
 
 extern "C" int pfa_focus_f32_dev(pfa_plan *p, const float *sig, const double *freq, void *out,
                                  int out_kind) {
+    PFA_TRY
     CK(float_check(p, sig, freq, out, out_kind));
     pfa_ctx *ctx = p->ctx;
     int st = PFA_OK;
@@ -626,11 +739,12 @@ extern "C" int pfa_focus_f32_dev(pfa_plan *p, const float *sig, const double *fr
     CK(launch_float_azimuth(p, 1, g1t, ldp, nullptr, A));
     CK(launch_float_fft_b(p, A, out, out_kind));
     return PFA_OK;
+    PFA_CATCH((p ? p->ctx : nullptr))
 }
 
 // ------------------------------------------------------------------ autofocus
 static int af_check(pfa_plan *p) {
-    if (!p || p->kind != PLAN_FLOAT) return PFA_ERR_STATE;
+    if (!p || !p->ctx || p->kind != PLAN_FLOAT) return PFA_ERR_STATE;
     if (p->flags & (PFA_FLOAT_FFT_BF16 | PFA_FLOAT_FFT_FP16))
         return pfa_fail(p->ctx, PFA_ERR_ARG, "autofocus: needs the FP32 FFT form");
     if (p->m2 > 4096) return pfa_fail(p->ctx, PFA_ERR_DIMS, "autofocus: padded cross size above 4096");
@@ -639,28 +753,40 @@ static int af_check(pfa_plan *p) {
 }
 
 extern "C" int pfa_af_checkpoint(pfa_plan *p, const float *sig, const double *freq) {
+    PFA_TRY
     CK(af_check(p));
     if (!sig) return pfa_fail(p->ctx, PFA_ERR_ARG, "af_checkpoint: null signal");
     if (!freq && !p->f0d) return pfa_fail(p->ctx, PFA_ERR_ARG, "af_checkpoint: plan has no compact frequencies; pass freq");
-    int st = PFA_OK;
     const int ldp = ldp32(p->m);
-    void *g1t = pfa_ws(p->ctx, "af_g1t", (size_t)p->n * ldp * 8, &st);
-    if (!g1t) return st;
-    CK(launch_float_range(p, sig, freq, g1t, ldp));
+    if (!p->af_g1t) {                             // the plan's own buffer: plans never share it
+        void *d = nullptr;
+        const hipError_t e = hipMalloc(&d, (size_t)p->n * ldp * 8);
+        if (e != hipSuccess) return pfa_hip_fail(p->ctx, e, "autofocus checkpoint");
+        try {
+            p->owned.push_back(d);
+        } catch (...) {
+            hipFree(d);
+            throw;
+        }
+        p->af_g1t = d;
+    }
+    p->af_ready = false;
+    CK(launch_float_range(p, sig, freq, p->af_g1t, ldp));
     p->af_ready = true;
     return PFA_OK;
+    PFA_CATCH((p ? p->ctx : nullptr))
 }
 
 static int af_buffers(pfa_plan *p, void **g1t, void **rot, const float *phi) {
     int st = PFA_OK;
-    *g1t = pfa_ws(p->ctx, "af_g1t", (size_t)p->n * ldp32(p->m) * 8, &st);
-    if (!*g1t) return st;
+    *g1t = p->af_g1t;
     *rot = pfa_ws(p->ctx, "af_rot", (size_t)p->m * 8, &st);
     if (!*rot) return st;
     return launch_af_rot(p, phi, *rot);
 }
 
 extern "C" int pfa_af_forward(pfa_plan *p, const float *phi, float *img, double *stats) {
+    PFA_TRY
     CK(af_check(p));
     if (!p->af_ready) return pfa_fail(p->ctx, PFA_ERR_STATE, "af_forward: no checkpoint (call pfa_af_checkpoint)");
     if (!phi || !img || !stats) return pfa_fail(p->ctx, PFA_ERR_ARG, "af_forward: null pointer");
@@ -675,10 +801,12 @@ extern "C" int pfa_af_forward(pfa_plan *p, const float *phi, float *img, double 
     CK(launch_float_azimuth(p, 1, g1t, ldp32(p->m), nullptr, A, rot));
     CK(launch_float_fft_b(p, A, img, PFA_OUT_C64));
     return launch_af_stats(p, img, part, parts, stats);
+    PFA_CATCH((p ? p->ctx : nullptr))
 }
 
 extern "C" int pfa_af_backward(pfa_plan *p, const float *phi, const float *img,
                                const double *stats, float *grad) {
+    PFA_TRY
     CK(af_check(p));
     if (!p->af_ready) return pfa_fail(p->ctx, PFA_ERR_STATE, "af_backward: no checkpoint (call pfa_af_checkpoint)");
     if (!phi || !img || !stats || !grad) return pfa_fail(p->ctx, PFA_ERR_ARG, "af_backward: null pointer");
@@ -695,10 +823,12 @@ extern "C" int pfa_af_backward(pfa_plan *p, const float *phi, const float *img,
     CK(launch_af_az_adj(p, g1t, ldp32(p->m), rot, B, part, &nparts));
     if ((size_t)nparts > max_parts) return pfa_fail(p->ctx, PFA_ERR_STATE, "af_backward: partial buffer too small");
     return launch_af_grad_reduce(p, part, nparts, stats, grad);
+    PFA_CATCH((p ? p->ctx : nullptr))
 }
 
 extern "C" int pfa_focus_f32(pfa_plan *p, const float *sig, const double *freq, void *out,
                              int out_kind) {
+    PFA_TRY
     CK(float_check(p, sig, freq, out, out_kind));
     pfa_ctx *ctx = p->ctx;
     int st = PFA_OK;
@@ -718,10 +848,12 @@ extern "C" int pfa_focus_f32(pfa_plan *p, const float *sig, const double *freq, 
     CK(pfa_focus_f32_dev(p, (const float *)ds, dfreq, dout, out_kind));
     CK(pfa_io_d2h(ctx, out, dout, out_b));
     return PFA_OK;
+    PFA_CATCH((p ? p->ctx : nullptr))
 }
 
 extern "C" int pfa_resample_kspace_f32(pfa_plan *p, const float *sig, const double *freq,
                                        float *g2) {
+    PFA_TRY
     CK(float_check(p, sig, freq, g2, PFA_OUT_C64));
     pfa_ctx *ctx = p->ctx;
     int st = PFA_OK;
@@ -744,18 +876,22 @@ extern "C" int pfa_resample_kspace_f32(pfa_plan *p, const float *sig, const doub
     CK(launch_float_azimuth(p, 0, g1t, ldp, nullptr, dg));
     CK(pfa_io_d2h(ctx, g2, dg, in_b));
     return PFA_OK;
+    PFA_CATCH((p ? p->ctx : nullptr))
 }
 
 extern "C" int pfa_kr_extrema_dev(pfa_ctx *ctx, const double *freq, const double *pr, int m,
                                   int n, double *row_max, double *row_min) {
+    PFA_TRY
     if (!ctx) return PFA_ERR_ARG;
     if (m < 0 || n < 0 || ((size_t)m * n && (!freq || !pr || !row_max || !row_min)))
         return pfa_fail(ctx, PFA_ERR_ARG, "kr_extrema: bad argument");
     return launch_kr_extrema(ctx, freq, pr, m, n, row_max, row_min);
+    PFA_CATCH(ctx)
 }
 
 extern "C" int pfa_kr_extrema(pfa_ctx *ctx, const double *freq, const double *pr, int m, int n,
                               double *row_max, double *row_min) {
+    PFA_TRY
     if (!ctx) return PFA_ERR_ARG;
     if (m < 0 || n < 0 || ((size_t)m * n && (!freq || !pr || !row_max || !row_min)))
         return pfa_fail(ctx, PFA_ERR_ARG, "kr_extrema: bad argument");
@@ -771,10 +907,12 @@ extern "C" int pfa_kr_extrema(pfa_ctx *ctx, const double *freq, const double *pr
     CK(pfa_io_d2h(ctx, row_max, dp + m, (size_t)m * 8));
     CK(pfa_io_d2h(ctx, row_min, dp + 2 * m, (size_t)m * 8));
     return PFA_OK;
+    PFA_CATCH(ctx)
 }
 
 extern "C" int pfa_transform_f32(pfa_plan *p, const float *g2, void *out, int out_kind) {
-    if (!p || p->kind != PLAN_FLOAT) return PFA_ERR_STATE;
+    PFA_TRY
+    if (!p || !p->ctx || p->kind != PLAN_FLOAT) return PFA_ERR_STATE;
     pfa_ctx *ctx = p->ctx;
     if (!g2 || !out || (out_kind != PFA_OUT_C64 && out_kind != PFA_OUT_ABS_F32))
         return pfa_fail(ctx, PFA_ERR_ARG, "transform: bad argument");
@@ -792,4 +930,5 @@ extern "C" int pfa_transform_f32(pfa_plan *p, const float *g2, void *out, int ou
     CK(launch_float_fft_b(p, A, dout, out_kind));
     CK(pfa_io_d2h(ctx, out, dout, out_b));
     return PFA_OK;
+    PFA_CATCH((p ? p->ctx : nullptr))
 }

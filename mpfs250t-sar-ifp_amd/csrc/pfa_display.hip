@@ -403,15 +403,18 @@ static int launch_select_f32(pfa_ctx *ctx, const float *x, int64_t count, int64_
 
 // ---------------------------------------------------------------- entry points
 extern "C" int pfa_hist_u16_dev(pfa_ctx *ctx, const uint16_t *img, int64_t count, uint32_t *hist) {
+    PFA_TRY
     if (!ctx) return PFA_ERR_ARG;
     if (count < 0 || !hist || (count && !img)) return pfa_fail(ctx, PFA_ERR_ARG, "hist_u16: bad argument");
     pfa_timer_begin(ctx, PFA_TIMER_OTHER);
     const int st = launch_hist_u16(ctx, img, count, hist);
     pfa_timer_end(ctx, PFA_TIMER_OTHER);
     return st;
+    PFA_CATCH(ctx)
 }
 
 extern "C" int pfa_hist_u16(pfa_ctx *ctx, const uint16_t *img, int64_t count, uint32_t *hist) {
+    PFA_TRY
     if (!ctx) return PFA_ERR_ARG;
     if (count < 0 || !hist || (count && !img)) return pfa_fail(ctx, PFA_ERR_ARG, "hist_u16: bad argument");
     int st = PFA_OK;
@@ -423,10 +426,12 @@ extern "C" int pfa_hist_u16(pfa_ctx *ctx, const uint16_t *img, int64_t count, ui
     CK(pfa_hist_u16_dev(ctx, di, count, dh));
     CK(pfa_io_d2h(ctx, hist, dh, 65536 * 4));
     return PFA_OK;
+    PFA_CATCH(ctx)
 }
 
 extern "C" int pfa_lut_u16_u8_dev(pfa_ctx *ctx, const uint16_t *img, int64_t count, const uint8_t *lut,
                                   int lo, int nb, uint8_t *out) {
+    PFA_TRY
     if (!ctx) return PFA_ERR_ARG;
     if (count < 0 || nb < 1 || nb > 65536 || lo < 0 || lo > 65535 || !lut || (count && (!img || !out)))
         return pfa_fail(ctx, PFA_ERR_ARG, "lut_u16_u8: bad argument");
@@ -434,10 +439,12 @@ extern "C" int pfa_lut_u16_u8_dev(pfa_ctx *ctx, const uint16_t *img, int64_t cou
     const int st = launch_lut_u16_u8(ctx, img, count, lut, lo, nb, out);
     pfa_timer_end(ctx, PFA_TIMER_OTHER);
     return st;
+    PFA_CATCH(ctx)
 }
 
 extern "C" int pfa_lut_u16_u8(pfa_ctx *ctx, const uint16_t *img, int64_t count, const uint8_t *lut,
                               int lo, int nb, uint8_t *out) {
+    PFA_TRY
     if (!ctx) return PFA_ERR_ARG;
     if (count < 0 || nb < 1 || nb > 65536 || lo < 0 || lo > 65535 || !lut || (count && (!img || !out)))
         return pfa_fail(ctx, PFA_ERR_ARG, "lut_u16_u8: bad argument");
@@ -454,18 +461,22 @@ extern "C" int pfa_lut_u16_u8(pfa_ctx *ctx, const uint16_t *img, int64_t count, 
     CK(pfa_lut_u16_u8_dev(ctx, di, count, dl, lo, nb, dout));
     CK(pfa_io_d2h(ctx, out, dout, (size_t)count));
     return PFA_OK;
+    PFA_CATCH(ctx)
 }
 
 extern "C" int pfa_stretch_u8_dev(pfa_ctx *ctx, const float *x, int64_t count, float d, uint8_t *out) {
+    PFA_TRY
     if (!ctx) return PFA_ERR_ARG;
     if (count < 0 || (count && (!x || !out))) return pfa_fail(ctx, PFA_ERR_ARG, "stretch_u8: bad argument");
     pfa_timer_begin(ctx, PFA_TIMER_OTHER);
     const int st = launch_stretch_u8(ctx, x, count, d, out);
     pfa_timer_end(ctx, PFA_TIMER_OTHER);
     return st;
+    PFA_CATCH(ctx)
 }
 
 extern "C" int pfa_stretch_u8(pfa_ctx *ctx, const float *x, int64_t count, float d, uint8_t *out) {
+    PFA_TRY
     if (!ctx) return PFA_ERR_ARG;
     if (count < 0 || (count && (!x || !out))) return pfa_fail(ctx, PFA_ERR_ARG, "stretch_u8: bad argument");
     if (!count) return PFA_OK;
@@ -478,10 +489,12 @@ extern "C" int pfa_stretch_u8(pfa_ctx *ctx, const float *x, int64_t count, float
     CK(pfa_stretch_u8_dev(ctx, dx, count, d, dout));
     CK(pfa_io_d2h(ctx, out, dout, (size_t)count));
     return PFA_OK;
+    PFA_CATCH(ctx)
 }
 
 extern "C" int pfa_select_f32_dev(pfa_ctx *ctx, const float *x, int64_t count, int64_t k0, int64_t k1,
                                   float *v0, float *v1, int64_t *nan_count) {
+    PFA_TRY
     if (!ctx) return PFA_ERR_ARG;
     if (count < 1 || !x || !v0 || !v1 || k0 < 0 || k1 < 0 || k0 >= count || k1 >= count)
         return pfa_fail(ctx, PFA_ERR_ARG, "select_f32: ranks must lie in [0, count)");
@@ -500,10 +513,12 @@ extern "C" int pfa_select_f32_dev(pfa_ctx *ctx, const float *x, int64_t count, i
     *v1 = key_to_float(h.key[1]);
     if (nan_count) *nan_count = (int64_t)h.nan_count;
     return PFA_OK;
+    PFA_CATCH(ctx)
 }
 
 extern "C" int pfa_select_f32(pfa_ctx *ctx, const float *x, int64_t count, int64_t k0, int64_t k1,
                               float *v0, float *v1, int64_t *nan_count) {
+    PFA_TRY
     if (!ctx) return PFA_ERR_ARG;
     if (count < 1 || !x || !v0 || !v1 || k0 < 0 || k1 < 0 || k0 >= count || k1 >= count)
         return pfa_fail(ctx, PFA_ERR_ARG, "select_f32: ranks must lie in [0, count)");
@@ -512,4 +527,5 @@ extern "C" int pfa_select_f32(pfa_ctx *ctx, const float *x, int64_t count, int64
     if (!dx) return st;
     CK(pfa_io_h2d(ctx, dx, x, (size_t)count * 4));
     return pfa_select_f32_dev(ctx, dx, count, k0, k1, v0, v1, nan_count);
+    PFA_CATCH(ctx)
 }

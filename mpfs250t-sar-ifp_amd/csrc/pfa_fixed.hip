@@ -720,7 +720,7 @@ int launch_fixed_azimuth_fft(pfa_plan *p, const void *g1t, int ldp, void *ya, in
                              int *emax_a, bool do_fft, bool do_window) {
     pfa_ctx *ctx = p->ctx;
     int st = PFA_OK;
-    const short2 *tw = twiddles16(ctx, p->grid, &st);
+    const short2 *tw = p->tw16 ? (const short2 *)p->tw16 : twiddles16(ctx, p->grid, &st);
     if (!tw) return st;
     const int L = log2_exact(p->grid);
     pfa_timer_begin(ctx, PFA_TIMER_AZIMUTH);
@@ -748,7 +748,7 @@ int launch_fixed_fft_b(pfa_plan *p, const void *ya, const int32_t *ea, const int
                        void *yb, int32_t *eb, int *emax_b) {
     pfa_ctx *ctx = p->ctx;
     int st = PFA_OK;
-    const short2 *tw = twiddles16(ctx, p->grid, &st);
+    const short2 *tw = p->tw16 ? (const short2 *)p->tw16 : twiddles16(ctx, p->grid, &st);
     if (!tw) return st;
     const int L = log2_exact(p->grid);
     pfa_timer_begin(ctx, PFA_TIMER_FFT_B);

@@ -27,17 +27,18 @@ constexpr unsigned MAX_THREADS = 16;        // host threads copying (caller incl
 // the calling thread and returns when every part is done.
 class CopyPool {
   public:
+    // Throws (std::system_error, std::bad_alloc) when a thread cannot be
+    // started; the threads already running are stopped and joined first.
     explicit CopyPool(unsigned nthreads) {
-        for (unsigned i = 1; i < nthreads; ++i) th_.emplace_back([this, i] { worker(i); });
-    }
-    ~CopyPool() {
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            stop_ = true;
+        try {
+            th_.reserve(nthreads);
+            for (unsigned i = 1; i < nthreads; ++i) th_.emplace_back([this, i] { worker(i); });
+        } catch (...) {
+            shutdown();
+            throw;
         }
-        cv_.notify_all();
-        for (auto &t : th_) t.join();
     }
+    ~CopyPool() { shutdown(); }
     void copy(void *dst, const void *src, size_t bytes) {
         const unsigned parts = (unsigned)th_.size() + 1;
         if (parts == 1 || bytes < (1u << 20)) {
@@ -60,6 +61,15 @@ class CopyPool {
     }
 
   private:
+    void shutdown() noexcept {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : th_)
+            if (t.joinable()) t.join();
+    }
     void part(unsigned i) {
         const size_t per = ((bytes_ + parts_ - 1) / parts_ + 4095) & ~(size_t)4095;
         const size_t lo = std::min(bytes_, (size_t)i * per), hi = std::min(bytes_, lo + per);
@@ -133,7 +143,11 @@ static pfa_hostio *hostio(pfa_ctx *ctx) {
              hipEventCreateWithFlags(&io->ev[k], hipEventDisableTiming) == hipSuccess;
     if (ok) {
         const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-        io->pool = new (std::nothrow) CopyPool(std::min(MAX_THREADS, hw));
+        try {
+            io->pool = new CopyPool(std::min(MAX_THREADS, hw));
+        } catch (...) {                        // no threads or no memory: copies go direct
+            io->pool = nullptr;
+        }
         ok = io->pool != nullptr;
     }
     if (!ok) {

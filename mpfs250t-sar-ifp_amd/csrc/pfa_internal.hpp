@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <map>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -37,9 +38,11 @@ struct pfa_ctx {
     unsigned *d_maxbits = nullptr;        // quantiser max |x| as float bits
     pfa_hostio *io = nullptr;             // created on the first large host copy
     bool io_failed = false;               // staging unavailable: copies go direct
+    std::set<pfa_plan *> plans;           // live plans (orphaned by pfa_destroy)
 };
 
 enum PlanKind { PLAN_FIXED = 1, PLAN_FLOAT = 2 };
+
 
 struct pfa_plan {
     PlanKind kind;
@@ -50,6 +53,7 @@ struct pfa_plan {
     float *krp = nullptr, *kcp = nullptr;
     int32_t *order = nullptr, *hamr = nullptr, *hamc = nullptr;
     int16_t *sinc = nullptr;
+    const void *tw16 = nullptr;           // grid-point Q15 twiddles (short2[grid / 2])
     // float-mode device tables
     int sgn = -1, m2 = 0, n2 = 0;
     double *pr64 = nullptr, *a64 = nullptr, *tan_s64 = nullptr, *KR = nullptr, *KC = nullptr;
@@ -62,13 +66,26 @@ struct pfa_plan {
     // b = 0..tb_bins (a starting point only; the kernels fix brackets up exactly)
     int32_t *tan_bin = nullptr;
     bool af_ready = false;                // autofocus checkpoint (range pass) computed
+    void *af_g1t = nullptr;               // that checkpoint: the plan's own corner-turned range pass
     int tb_bins = 0;
     double tb_lo = 0.0, tb_inv_w = 0.0;
     std::vector<void *> owned;            // device allocations of the plan
 };
 
 // ---------------------------------------------------------------- helpers
+// Exceptions never cross the C ABI: every entry point runs its body between
+// PFA_TRY and PFA_CATCH(ctx).  Host memory exhaustion (std::bad_alloc) and a
+// host resource failure such as a thread that cannot start (std::system_error)
+// return PFA_ERR_NOMEM; any other exception returns PFA_ERR_STATE.
+int pfa_exception(pfa_ctx *ctx) noexcept;
+#define PFA_TRY try {
+#define PFA_CATCH(ctx)                     \
+    }                                      \
+    catch (...) {                          \
+        return pfa_exception(ctx);         \
+    }
 int pfa_fail(pfa_ctx *ctx, int code, const std::string &msg);
+int pfa_fail(pfa_ctx *ctx, int code, const char *msg) noexcept;
 int pfa_hip_fail(pfa_ctx *ctx, hipError_t e, const char *what);
 void *pfa_ws(pfa_ctx *ctx, const char *slot, size_t bytes, int *status);
 void pfa_timer_begin(pfa_ctx *ctx, int kind);

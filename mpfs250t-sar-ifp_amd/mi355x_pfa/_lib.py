@@ -53,6 +53,7 @@ PROTOS = {
     "pfa_fft_bfp_perrow": [_vp, _vp, _i, _i, _vp, _vp],
     "pfa_window_fixed": [_vp, _vp, _i, _i, _vp, _vp, _vp],
     "pfa_detect_fixed": [_vp, _vp, _i64, _vp],
+    "pfa_detect_fixed_dev": [_vp, _vp, _i64, _vp],
     "pfa_plan_fixed": [_vp, C.POINTER(_vp), _i, _i, _i] + [_vp] * 10 + [_i],
     "pfa_resample_fixed": [_vp, _vp, _vp],
     "pfa_form_image_fixed": [_vp, _vp, _vp, _vp, _vp],
@@ -62,6 +63,9 @@ PROTOS = {
     "pfa_focus_f32_dev": [_vp, _vp, _vp, _vp, _i],
     "pfa_resample_kspace_f32": [_vp, _vp, _vp, _vp],
     "pfa_transform_f32": [_vp, _vp, _vp, _i],
+    "pfa_float_geometry": [_i, _i] + [_vp] * 13,
+    "pfa_plan_float_pfa": [_vp, C.POINTER(_vp), _i, _i, _i] + [_vp] * 5 + [_i, _vp, _i, _vp],
+    "pfa_pfa_f32": [_vp, _vp, _i, _i, _vp, _vp, _vp, _i, _i, _vp, _i, _vp],
     "pfa_kr_extrema": [_vp, _vp, _vp, _i, _i, _vp, _vp],
     "pfa_kr_extrema_dev": [_vp, _vp, _vp, _i, _i, _vp, _vp],
     "pfa_hist_u16": [_vp, _vp, _i64, _vp],

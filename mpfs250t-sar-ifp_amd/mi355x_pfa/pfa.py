@@ -101,7 +101,7 @@ def float_geometry(ax, ay, freq=None, f0=None, df=None, n=None, device=0, k0=0,
     kc_max = float(np.max(np.abs(KR[[0, -1]])) * np.max(np.abs(tan_phi)))
     m = ax.size
     KC = np.linspace(-kc_max, kc_max, m)
-    order = np.argsort(tan_phi, kind="stable")
+    order = np.argsort(tan_phi)
     dr = 1.0 / (KR[-1] - KR[0]) if KR[-1] != KR[0] else float("nan")
     dc = 1.0 / (KC[-1] - KC[0]) if KC[-1] != KC[0] else float("nan")
     return {"pr": pr, "pc": pc, "tan_phi": tan_phi, "order": order.astype(np.int32),

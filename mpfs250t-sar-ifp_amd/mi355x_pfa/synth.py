@@ -115,10 +115,11 @@ def _phase_error(spec: SceneSpec, rs):
     return quad + walk
 
 
-def make_scene(spec: SceneSpec, device=None, chunk_targets=32) -> Scene:
+def make_scene(spec: SceneSpec, device=None, chunk_targets=32, signal=True) -> Scene:
     """Build the scene.  ``device=None`` synthesises the signal with numpy
     (float64 phases); a torch device string synthesises it there in float64
-    (used for the large configurations)."""
+    (used for the large configurations).  ``signal=False`` builds the
+    geometry, scatterers and phase error only (``sig`` stays None)."""
     rs = np.random.RandomState(spec.seed)
     apc, srp, sc0, scss = _geometry(spec)
     pos, amp = _scatterers(spec, rs)
@@ -128,6 +129,8 @@ def make_scene(spec: SceneSpec, device=None, chunk_targets=32) -> Scene:
                   np.array([1.0, 0.0, 0.0]), np.array([0.0, 1.0, 0.0]), -1,
                   pos, amp, perr)
     m, n = spec.pulses, spec.samples
+    if not signal:
+        return scene
     # range difference per (pulse, scatterer)
     r0 = np.linalg.norm(apc - srp, axis=1)                              # (m,)
     dR = np.linalg.norm(apc[:, None, :] - pos[None, :, :], axis=2) - r0[:, None]

@@ -75,9 +75,9 @@ def fixed_tables(tables: dict, m: int, n: int, grid: int) -> dict:
         "df": np.ascontiguousarray((freq[:, 1] - freq[:, 0]).astype(np.float32)),
         "pr": (ax * (mx / nrm) + ay * (my / nrm)).astype(np.float32),
     }
-    order = np.argsort(tan_phi, kind="stable")
+    order = np.argsort(tan_phi)
     out["tan_s"] = np.ascontiguousarray(tan_phi[order].astype(np.float32))
-    out["inv_order"] = np.argsort(order, kind="stable").astype(np.int32)
+    out["inv_order"] = np.argsort(order).astype(np.int32)
     pad_r = float(KR.max() + (KR.max() - KR.min()) + 1.0)
     pad_c = float(KC.max() + (KC.max() - KC.min()) + 1.0)
     krp = np.full(grid, pad_r, np.float32)

@@ -119,9 +119,9 @@ def fixed_tables(ax, ay, freq, KR, KC, tan_phi, m, n, grid):
     dx, dy = ax.mean(), ay.mean()
     dn = math.hypot(dx, dy)
     pr = (ax * (dx / dn) + ay * (dy / dn)).astype(np.float32)
-    order = np.argsort(tan_phi, kind="stable")
+    order = np.argsort(tan_phi)
     tan_s = tan_phi[order].astype(np.float32)
-    inv_order = np.argsort(order, kind="stable")
+    inv_order = np.argsort(order)
     oob_r = float(KR.max() + (KR.max() - KR.min()) + 1.0)
     oob_c = float(KC.max() + (KC.max() - KC.min()) + 1.0)
     krp = np.full(grid, oob_r, np.float32); krp[:n] = KR

@@ -57,7 +57,7 @@ def geometry(freq, ax, ay):
     KR = np.linspace(lo, hi, n)
     kc_max = np.abs(KR[:, None] * tan_phi[None, :]).max()
     KC = np.linspace(-kc_max, kc_max, m)
-    order = np.argsort(tan_phi, kind="stable")
+    order = np.argsort(tan_phi)
     dr = 1.0 / (KR[-1] - KR[0]) if KR[-1] != KR[0] else float("nan")
     dc = 1.0 / (KC[-1] - KC[0]) if KC[-1] != KC[0] else float("nan")
     geo = {"dc": dc, "dr": dr, "dhat": (dx, dy), "chat": (cx, cy)}

@@ -43,8 +43,12 @@ def test_resample_kspace_matches_reference(gp, name):
     ref = g["g2"]
     err = np.max(np.abs(g2.astype(np.complex128) - ref)) / np.max(np.abs(ref))
     assert err <= 1e-6, err
-    assert np.allclose([geo["dc"], geo["dr"], *geo["dhat"], *geo["chat"]][:2],
-                       [g["geo"][0] * 1.0, g["geo"][1] * 1.0], rtol=1.0)   # unpadded here
+    # resample_kspace's geo is the unpadded one; focus scales it by n / n2 and
+    # m / m2 (form_image_pfa.py:237-238) into the fixture's (pfa's) geo
+    m, n = ref.shape
+    m2, n2 = 1 << (m - 1).bit_length(), 1 << (n - 1).bit_length()
+    assert geo["dr"] * (n / n2) == g["geo"][1] and geo["dc"] * (m / m2) == g["geo"][0]
+    assert [*geo["dhat"], *geo["chat"]] == list(g["geo"][2:])
 
 
 @pytest.mark.parametrize("name", ["small", "rect"])
