@@ -140,6 +140,17 @@ int pfa_form_image_fixed(pfa_plan *plan, const int16_t *sig_iq, uint16_t *out,
                          int32_t *exp_a, int32_t *exp_b);
 int pfa_form_image_fixed_dev(pfa_plan *plan, const int16_t *sig_iq, uint16_t *out,
                              int32_t *exp_a, int32_t *exp_b);
+/* pfa_form_image_fixed_dev without any device allocation.  No counterpart in
+ * the reference: for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Its
+ * intermediates live in a static device array of the library (about 32 MB
+ * per GPU, held until the process ends), so grid <= PFA_NOALLOC_MAX_GRID
+ * (else PFA_ERR_DIMS); the result equals pfa_form_image_fixed_dev's bit for
+ * bit.  Calls on one device share that array: calls that overlap on one GPU
+ * (two streams, two host threads) race.  No timer events are recorded. */
+#define PFA_NOALLOC_MAX_GRID 2048
+int pfa_form_image_fixed_dev_noalloc(pfa_plan *plan, const int16_t *sig_iq, uint16_t *out,
+                                     int32_t *exp_a, int32_t *exp_b);
 
 /* ------------------------------------------------------------------ float mode
  * form_image_pfa.py in FP32.  The O(m + n) scene geometry (form_image_pfa.py

@@ -78,14 +78,14 @@ static hipEvent_t take_event(pfa_ctx *ctx) {
 }
 
 void pfa_timer_begin(pfa_ctx *ctx, int kind) {
-    if (!ctx->timing) return;
+    if (!ctx->timing || ctx->no_timers) return;
     TimerMark mk{kind, take_event(ctx), take_event(ctx)};
     hipEventRecord(mk.start, ctx->stream);
     ctx->marks.push_back(mk);
 }
 
 void pfa_timer_end(pfa_ctx *ctx, int kind) {
-    if (!ctx->timing || ctx->marks.empty()) return;
+    if (!ctx->timing || ctx->no_timers || ctx->marks.empty()) return;
     TimerMark &mk = ctx->marks.back();
     if (mk.kind == kind) hipEventRecord(mk.stop, ctx->stream);
 }
@@ -117,6 +117,13 @@ extern "C" int pfa_create(pfa_ctx **out, int device) {
         hipStreamDestroy(ctx->own_stream);
         delete ctx;
         return PFA_ERR_NOMEM;
+    }
+    if (pfa_scratch_init(ctx) != PFA_OK) {
+        hipFree(ctx->d_counters);
+        hipFree(ctx->d_maxbits);
+        hipStreamDestroy(ctx->own_stream);
+        delete ctx;
+        return PFA_ERR_HIP;
     }
     *out = ctx;
     return PFA_OK;

@@ -39,6 +39,8 @@ struct pfa_ctx {
     pfa_hostio *io = nullptr;             // created on the first large host copy
     bool io_failed = false;               // staging unavailable: copies go direct
     std::set<pfa_plan *> plans;           // live plans (orphaned by pfa_destroy)
+    bool no_timers = false;               // set while an allocation-free call runs
+    void *scratch = nullptr;              // this device's copy of the static scratch array
 };
 
 enum PlanKind { PLAN_FIXED = 1, PLAN_FLOAT = 2 };
@@ -94,6 +96,9 @@ void pfa_timer_end(pfa_ctx *ctx, int kind);
 // through pinned buffers on a copy stream (pfa_hostio.cpp).  h2d leaves the
 // context stream waiting for the data; d2h returns when the host has it.
 int pfa_io_h2d(pfa_ctx *ctx, void *dst, const void *src, size_t bytes);
+// Address of this device's static scratch array (pfa_noalloc.hip); called by
+// pfa_create with the context's device current.
+int pfa_scratch_init(pfa_ctx *ctx);
 int pfa_io_d2h(pfa_ctx *ctx, void *dst, const void *src, size_t bytes);
 void pfa_hostio_destroy(pfa_ctx *ctx);
 

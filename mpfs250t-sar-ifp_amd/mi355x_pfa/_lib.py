@@ -58,6 +58,7 @@ PROTOS = {
     "pfa_resample_fixed": [_vp, _vp, _vp],
     "pfa_form_image_fixed": [_vp, _vp, _vp, _vp, _vp],
     "pfa_form_image_fixed_dev": [_vp, _vp, _vp, _vp, _vp],
+    "pfa_form_image_fixed_dev_noalloc": [_vp, _vp, _vp, _vp, _vp],
     "pfa_plan_float": [_vp, C.POINTER(_vp), _i, _i, _i] + [_vp] * 10 + [_i],
     "pfa_focus_f32": [_vp, _vp, _vp, _vp, _i],
     "pfa_focus_f32_dev": [_vp, _vp, _vp, _vp, _i],
