@@ -430,11 +430,16 @@ extern "C" int pfa_plan_fixed(pfa_ctx *ctx, pfa_plan **out, int m, int n, int gr
         return pfa_fail(ctx, PFA_ERR_DIMS, "plan_fixed: need 2 <= m, n <= grid");
     if ((flags & (PFA_FIXED_RANGE_SINC | PFA_FIXED_AZ_SINC)) && !sinc_tab)
         return pfa_fail(ctx, PFA_ERR_ARG, "plan_fixed: sinc flags need the 256x32 tap table");
+    bool split = sinc_tab != nullptr;                // each half of every phase: sum |c| <= 65535
     if (sinc_tab) {
         for (int ph = 0; ph < 256; ++ph) {
-            int sum = 0;
-            for (int t = 0; t < 32; ++t) sum += sinc_tab[ph * 32 + t];
+            int sum = 0, a0 = 0, a1 = 0;
+            for (int t = 0; t < 32; ++t) {
+                sum += sinc_tab[ph * 32 + t];
+                (t < 16 ? a0 : a1) += std::abs((int)sinc_tab[ph * 32 + t]);
+            }
             if (sum != 32768) return pfa_fail(ctx, PFA_ERR_GEOMETRY, "sinc table phase does not sum to 2^15");
+            split = split && a0 <= 65535 && a1 <= 65535;
         }
     }
     for (int i = 0; i < m; ++i)
@@ -465,7 +470,7 @@ extern "C" int pfa_plan_fixed(pfa_ctx *ctx, pfa_plan **out, int m, int n, int gr
     if (!p) return PFA_ERR_NOMEM;
     p->kind = PLAN_FIXED;
     p->ctx = ctx;
-    p->m = m; p->n = n; p->grid = grid; p->flags = flags;
+    p->m = m; p->n = n; p->grid = grid; p->flags = flags | (split ? FIXED_SINC_SPLIT : 0);
     int st = PFA_OK;
     bool ok = (p->a32 = upload(p, a32.data(), m, &st)) && (p->f0 = upload(p, f0, m, &st)) &&
               (p->df = upload(p, df, m, &st)) && (p->tan_s = upload(p, tan_s, m, &st)) &&

@@ -57,7 +57,9 @@ __device__ __forceinline__ int wave_max_dpp(int v) {
 template <int L>
 struct Fft {
     static constexpr int N = 1 << L;
-    static constexpr int NT = (N / 2 < 256) ? N / 2 : 256;
+    // threads per line: 16 elements each from 1024 points up (512 threads at
+    // 8192, so a CU holds 16 waves of two lines), two each below
+    static constexpr int NT = N >= 1024 ? (N / 16 < 512 ? N / 16 : 512) : (N / 2 < 256 ? N / 2 : 256);
     static constexpr int E = N / NT;
     static constexpr int P = ilog2c(E);
     static constexpr int NB = NT < 64 ? 64 : NT;
@@ -121,40 +123,48 @@ struct Fft {
                 }
             }
             constexpr int q = (S - 1) - nb;
-            constexpr int h = 1 << (S - 1);
-            int lmax = 0;
+            // twiddle index of local butterfly l: j = idx(t, l, nb) & (h - 1) =
+            // (t's low nb bits) + ((l's bits below q) << nb), so the thread's
+            // twiddles lie at one base plus compile-time offsets
+            const short2 *twb = tw + ((t & ((1 << nb) - 1)) << (L - S));
+            int vmax = 0, vmin = 0;
             if (act) {
 #pragma unroll
                 for (int l = 0; l < E; ++l) {
                     if (l & (1 << q)) continue;
                     const int l1 = l | (1 << q);
-                    const int j = idx(t, l, nb) & (h - 1);
-                    const short2 w = tw[j << (L - S)];
+                    const short2 w = twb[((l & ((1 << q) - 1)) << nb) << (L - S)];
                     const int br = xr[l1], bi = xi[l1];
-                    // |w| < 2^15 and |b| <= 2^15 at a stage's start: 24-bit products are exact
+                    // |w| <= 2^15 and |b| <= 2^15 at a stage's start: 24-bit products are exact
                     const int tr = sat16((__mul24(w.x, br) - __mul24(w.y, bi)) >> 15);
                     const int ti = sat16((__mul24(w.x, bi) + __mul24(w.y, br)) >> 15);
                     const int ar = xr[l], ai = xi[l];
-                    xr[l] = ar + tr;
-                    xi[l] = ai + ti;
-                    xr[l1] = ar - tr;
-                    xi[l1] = ai - ti;
-                    lmax = max(lmax, max(max(abs(ar + tr), abs(ai + ti)),
-                                         max(abs(ar - tr), abs(ai - ti))));
+                    const int y0r = ar + tr, y0i = ai + ti, y1r = ar - tr, y1i = ai - ti;
+                    xr[l] = y0r;
+                    xi[l] = y0i;
+                    xr[l1] = y1r;
+                    xi[l1] = y1i;
+                    vmax = max(max(vmax, y0r), max(y0i, max(y1r, y1i)));
+                    vmin = min(min(vmin, y0r), min(y0i, min(y1r, y1i)));
                 }
             }
-            lmax = wave_max_dpp(lmax);
+            int lmax = wave_max_dpp(max(vmax, -vmin));   // max |y| of the wave
             if ((t & 63) == 63) red[(S & 1) * NW + (t >> 6)] = lmax;
             __syncthreads();
             int mx = red[(S & 1) * NW];
 #pragma unroll
             for (int w = 1; w < NW; ++w) mx = max(mx, red[(S & 1) * NW + w]);
             const int sh = mx <= 32766 ? 0 : (mx <= 65533 ? 1 : 2);
+            // The reference clips floor(y / 2^sh) to int16.  That clip never
+            // acts: a stage's inputs are int16 and t is saturated, so |y| <=
+            // 65536; sh = 0 leaves |y| <= 32766, sh = 1 gives floor(y / 2) in
+            // [-32767, 32766] (|y| <= 65533), sh = 2 floor(y / 4) in
+            // [-16384, 16383].
             if (sh && act) {
 #pragma unroll
                 for (int l = 0; l < E; ++l) {
-                    xr[l] = sat16(xr[l] >> sh);
-                    xi[l] = sat16(xi[l] >> sh);
+                    xr[l] >>= sh;
+                    xi[l] >>= sh;
                 }
             }
             e += sh;
@@ -313,6 +323,42 @@ __device__ __forceinline__ short2 sinc_q15w(const short2 *row, int idx, const in
                        (short)max(min(ai >> 15, (int64_t)32767), (int64_t)-32768));
 }
 
+// The same gather with one dot product per tap pair and component: the
+// plan has checked that every phase's taps 0..15 and taps 16..31 each have
+// sum |c| <= 65535, so each group's partial sums stay within
+// 65535 * 2^15 < 2^31 for any int16 samples and the two int32 accumulators
+// of a component are exact.  Their sum plus 2^14, shifted by 15, is formed
+// from 15-bit halves so nothing wraps.  One byte permute per component and
+// pair builds the (x[2k], x[2k+1]) int16 lanes.
+constexpr unsigned SEL_RE = 0x05040100u;      // (re0, re1)
+constexpr unsigned SEL_IM = 0x07060302u;      // (im0, im1)
+__device__ __forceinline__ int round_pair15(int a, int b) {
+    const int lo = ((a & 32767) + (b & 32767) + 16384) >> 15;
+    return sat16((a >> 15) + (b >> 15) + lo);
+}
+__device__ __forceinline__ short2 sinc_q15s(const short2 *row, int idx, const int16_t *taps) {
+    const uint4 *tw = reinterpret_cast<const uint4 *>(taps);
+    uint4 w[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) w[i] = tw[i];
+    int r0 = 0, r1 = 0, i0 = 0, i1 = 0;
+    const unsigned *s = reinterpret_cast<const unsigned *>(row + idx - 15);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const unsigned cw = (&w[k >> 2].x)[k & 3];    // taps 2k (low half), 2k + 1
+        const unsigned s0 = s[2 * k], s1 = s[2 * k + 1];
+        const unsigned pr = byte_pair(s1, s0, SEL_RE), pi = byte_pair(s1, s0, SEL_IM);
+        if (k < 8) {
+            r0 = dot2(cw, pr, r0);
+            i0 = dot2(cw, pi, i0);
+        } else {
+            r1 = dot2(cw, pr, r1);
+            i1 = dot2(cw, pi, i1);
+        }
+    }
+    return make_short2((short)round_pair15(r0, r1), (short)round_pair15(i0, i1));
+}
+
 __device__ __forceinline__ int q15_weight(double w) {
     const double r = rint(w * 32768.0);
     return (int)fmin(fmax(r, 0.0), 32767.0);
@@ -327,14 +373,30 @@ __device__ __forceinline__ double xpos(double a, double f0, double df, int j) {
 // ------------------------------------------------------------------ pass 1
 constexpr int RT = 64;   // pulses per tile
 constexpr int QT = 64;   // range outputs per tile
+constexpr int SW = 112;  // staged source samples per pulse (64 outputs + window + slack)
+constexpr int SWP = SW + 1;  // odd pitch: the pulses of a wave start in different banks
 
+// Tile of RT sorted pulses x QT range outputs; thread t serves pulse t / 4 and
+// the 16 consecutive outputs 16 (t % 4) + u.  Three phases:
+//  (1) brackets and Q15 weights of the thread's outputs, walking the FP64
+//      positions with the two that bracket the previous output kept;
+//  (2) each pulse's source strip -- from its first window on, SW samples --
+//      staged in LDS by coalesced loads (the four threads of a pulse agree on
+//      its start with two lane swaps);
+//  (3) the gathers from the strip (from the row in global memory for the
+//      rare window that leaves it), then the tile transposed through LDS so
+//      g1t rows get whole 256-byte runs.
+template <bool SPLIT>
 __global__ __launch_bounds__(256) void k_range_resample(
     const short2 *__restrict__ sig, int m, int n, int grid, const float *__restrict__ a32,
     const float *__restrict__ f0v, const float *__restrict__ dfv,
     const int32_t *__restrict__ order, const float *__restrict__ krp,
     const int16_t *__restrict__ sinc, int flags, short2 *__restrict__ g1t, int ldp) {
-    __shared__ short2 tile[QT][RT + 1];
+    static_assert(QT * (RT + 1) <= RT * SWP, "the output tile fits over the strips");
+    __shared__ short2 strip[RT * SWP];             // source strips, then the output tile
     __shared__ uint4 stl[256 * 4];                 // the 256 x 32 Q15 tap table, 16 KB
+    __shared__ int lo_s[RT], row_s[RT];
+    auto tile = reinterpret_cast<short2 (*)[RT + 1]>(&strip[0]);
     const int p0 = blockIdx.x * RT, q0 = blockIdx.y * QT;
     const int t = threadIdx.x;
     const int pl = t >> 2, ch = t & 3;
@@ -342,52 +404,131 @@ __global__ __launch_bounds__(256) void k_range_resample(
     const bool use_sinc = flags & PFA_FIXED_RANGE_SINC;
     if (use_sinc)
         for (int i = t; i < 256 * 4; i += 256) stl[i] = reinterpret_cast<const uint4 *>(sinc)[i];
-    __syncthreads();
     const int16_t *sincl = reinterpret_cast<const int16_t *>(stl);
     const bool f32 = !(flags & PFA_FIXED_LERP_EXACT);
+    // (1) brackets: code = idx << 15 | wq (idx < 2^15, wq < 2^15), -1 = zero output
+    constexpr int PER = QT / 4;
+    int code[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) code[u] = -1;
+    int i = 0;
+    int lo = 1 << 30;
     if (p < m) {
-        const int i = order[p];
+        i = order[p];
         const double a = a32[i], f0 = f0v[i], df = dfv[i];
-        const short2 *row = sig + (size_t)i * n;
         const bool asc = xpos(a, f0, df, n - 1) >= xpos(a, f0, df, 0);
-        // ascending view xa(u) = x(asc ? u : n-1-u); c = #{xa < Q}
+        // ascending view xa(u) = x(asc ? u : n-1-u); c = #{u : xa(u) < Q}
         auto xa = [&](int u) { return xpos(a, f0, df, asc ? u : n - 1 - u); };
-        int c = -1;
-        for (int u = 0; u < QT / 4; ++u) {
-            const int q = q0 + ch * (QT / 4) + u;
+        int c = 0;
+        double xl = -INFINITY, xh = INFINITY;      // xa(c - 1), xa(c)
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int q = q0 + ch * PER + u;
             const double Q = krp[q];
-            if (c < 0) {                      // closed-form guess
+            if (u == 0) {                          // closed-form guess, then the exact count
                 const double js = (Q / a - f0) / df;
                 double g = asc ? js : (double)(n - 1) - js;
                 g = fmin(fmax(floor(g) + 1.0, 0.0), (double)n);
                 c = (g == g) ? (int)g : 0;
-            }
-            int steps = 0;
-            while (c < n && xa(c) < Q && steps < 64) { ++c; ++steps; }
-            while (c > 0 && !(xa(c - 1) < Q) && steps < 128) { --c; ++steps; }
-            if (steps >= 64) {                // far from the guess: binary search
-                int lo = 0, hi = n;
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (xa(mid) < Q) lo = mid + 1; else hi = mid;
+                int steps = 0;
+                xl = c > 0 ? xa(c - 1) : -INFINITY;
+                xh = c < n ? xa(c) : INFINITY;
+                while (c < n && xh < Q && steps < 64) { ++c; ++steps; xl = xh; xh = c < n ? xa(c) : INFINITY; }
+                while (c > 0 && !(xl < Q) && steps < 128) { --c; ++steps; xh = xl; xl = c > 0 ? xa(c - 1) : -INFINITY; }
+                if (steps >= 64) {                 // far from the guess: binary search
+                    int bl = 0, bh = n;
+                    while (bl < bh) {
+                        const int mid = (bl + bh) >> 1;
+                        if (xa(mid) < Q) bl = mid + 1; else bh = mid;
+                    }
+                    c = bl;
+                    xl = c > 0 ? xa(c - 1) : -INFINITY;
+                    xh = c < n ? xa(c) : INFINITY;
                 }
-                c = lo;
+            } else {                               // KRp is non-decreasing: walk up
+                int steps = 0;
+                while (c < n && xh < Q && steps < 64) { ++c; ++steps; xl = xh; xh = c < n ? xa(c) : INFINITY; }
+                if (c < n && xh < Q) {
+                    int bl = c, bh = n;
+                    while (bl < bh) {
+                        const int mid = (bl + bh) >> 1;
+                        if (xa(mid) < Q) bl = mid + 1; else bh = mid;
+                    }
+                    c = bl;
+                    xl = c > 0 ? xa(c - 1) : -INFINITY;
+                    xh = c < n ? xa(c) : INFINITY;
+                }
             }
             const int k = c - 1;
-            short2 v = make_short2(0, 0);
+            int cd = -1;
             if (k >= 0 && k <= n - 2) {
-                const double x0 = xa(k), x1 = xa(k + 1);
-                const double frac = div_bracket(Q - x0, x1 - x0);
+                const double frac = div_bracket(Q - xl, xh - xl);
                 const int idx = asc ? k : n - 2 - k;
                 const int wq = q15_weight(asc ? frac : 1.0 - frac);
-                if (use_sinc && idx >= 15 && idx <= n - 17)
-                    v = sinc_q15w(row, idx, sincl + (wq >> 7) * 32);
-                else
-                    v = lerp_q15(row, idx, wq, f32 && !use_sinc);
+                cd = idx << 15 | wq;
+                lo = min(lo, idx);
             }
-            tile[q - q0][pl] = v;
+            code[u] = cd;
         }
     }
+    // (2) the pulse's strip starts at its lowest window: min over its four threads
+    lo = min(lo, __shfl_xor(lo, 1, 64));
+    lo = min(lo, __shfl_xor(lo, 2, 64));
+    if (ch == 0) {
+        lo_s[pl] = max(0, min(lo - 15, n - SW));
+        row_s[pl] = p < m ? i : -1;
+    }
+    __syncthreads();
+    {                                              // all loads in flight, then the stores
+        constexpr int NL = RT * SW / 256;
+        static_assert(NL * 256 == RT * SW, "strips split evenly over the threads");
+        short2 v[NL];
+#pragma unroll
+        for (int k = 0; k < NL; ++k) {
+            const int f = t + 256 * k, pp = f / SW, u = f % SW;
+            const int r = row_s[pp], src = min(lo_s[pp] + u, n - 1);
+            v[k] = sig[(size_t)max(r, 0) * n + src];      // clamped: always in bounds
+        }
+#pragma unroll
+        for (int k = 0; k < NL; ++k) {
+            const int f = t + 256 * k, pp = f / SW, u = f % SW;
+            const bool ok = row_s[pp] >= 0 && lo_s[pp] + u < n;
+            strip[pp * SWP + u] = ok ? v[k] : make_short2(0, 0);
+        }
+    }
+    __syncthreads();
+    // (3) gathers
+    short2 out[PER];
+    {
+        const int base = lo_s[pl];
+        const short2 *sp = strip + pl * SWP;
+        const short2 *row = sig + (size_t)i * n;
+#pragma unroll 4
+        for (int u = 0; u < PER; ++u) {
+            const int cd = code[u];
+            short2 v = make_short2(0, 0);
+            if (cd >= 0) {
+                const int idx = cd >> 15, wq = cd & 0x7fff;
+                const int16_t *taps = sincl + (wq >> 7) * 32;
+                // strip reads stay LDS loads: the strip and the global row are
+                // separate calls, never one pointer that could be either
+                if (use_sinc && idx >= 15 && idx <= n - 17) {
+                    if (idx - 15 >= base && idx + 16 < base + SW)
+                        v = SPLIT ? sinc_q15s(sp, idx - base, taps) : sinc_q15w(sp, idx - base, taps);
+                    else
+                        v = SPLIT ? sinc_q15s(row, idx, taps) : sinc_q15w(row, idx, taps);
+                } else if (idx >= base && idx + 1 < base + SW) {
+                    v = lerp_q15(sp, idx - base, wq, f32 && !use_sinc);
+                } else {
+                    v = lerp_q15(row, idx, wq, f32 && !use_sinc);
+                }
+            }
+            out[u] = v;
+        }
+    }
+    __syncthreads();                               // strips dead: the tile takes over
+#pragma unroll
+    for (int u = 0; u < PER; ++u) tile[ch * PER + u][pl] = out[u];
     __syncthreads();
     for (int r = t; r < QT * RT; r += 256) {
         const int qq = r / RT, pp = r % RT;
@@ -410,7 +551,7 @@ __device__ __forceinline__ unsigned gather_code(int kind, int idx, int wq) {
     return (unsigned)kind << 30 | (unsigned)idx << 15 | (unsigned)wq;   // idx < 2^15, wq < 2^15
 }
 
-template <int L, bool DO_FFT>
+template <int L, bool DO_FFT, bool SPLIT>
 __global__ __launch_bounds__(Fft<L>::NB) void k_azimuth_fft(
     const short2 *__restrict__ g1t, int ldp, int m, const float *__restrict__ krp,
     const float *__restrict__ tan_s, const float *__restrict__ kcp,
@@ -445,36 +586,31 @@ __global__ __launch_bounds__(Fft<L>::NB) void k_azimuth_fft(
     const bool asc = srcv[m - 1] >= srcv[0];
     auto sa = [&](int u) { return srcv[asc ? u : m - 1 - u]; };
     const bool use_sinc = flags & PFA_FIXED_AZ_SINC;
-    // (A) brackets and weights
+    // (A) brackets and weights: c = #{u : sa(u) < Q}, with xl = sa(c - 1) and
+    // xh = sa(c) kept from one output to the next (KCp is non-decreasing)
     int c = -1;
+    float xl = -INFINITY, xh = INFINITY;
     for (int u = 0; u < PER; ++u) {
         const int q = t * PER + u;
         if (q >= G) break;
         const float Q = kcp[q];
-        if (c < 0) {
-            int lo = 0, hi = m;
+        int steps = 0;
+        if (c >= 0)
+            while (c < m && xh < Q && steps < 64) { ++c; ++steps; xl = xh; xh = c < m ? sa(c) : INFINITY; }
+        if (c < 0 || (c < m && xh < Q)) {        // first output, or far: bisect
+            int lo = c < 0 ? 0 : c, hi = m;
             while (lo < hi) {
                 const int mid = (lo + hi) >> 1;
                 if (sa(mid) < Q) lo = mid + 1; else hi = mid;
             }
             c = lo;
-        } else {
-            int steps = 0;
-            while (c < m && sa(c) < Q && steps < 64) { ++c; ++steps; }
-            while (c > 0 && !(sa(c - 1) < Q) && steps < 128) { --c; ++steps; }
-            if (steps >= 64) {
-                int lo = 0, hi = m;
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (sa(mid) < Q) lo = mid + 1; else hi = mid;
-                }
-                c = lo;
-            }
+            xl = c > 0 ? sa(c - 1) : -INFINITY;
+            xh = c < m ? sa(c) : INFINITY;
         }
         const int k = c - 1;
         unsigned code = 0;                                   // kind 0: zero output
         if (k >= 0 && k <= m - 2) {
-            const double x0 = sa(k), x1 = sa(k + 1);
+            const double x0 = xl, x1 = xh;
             const double frac = div_bracket((double)Q - x0, x1 - x0);
             const int idx = asc ? k : m - 2 - k;
             const int wq = q15_weight(asc ? frac : 1.0 - frac);
@@ -496,7 +632,8 @@ __global__ __launch_bounds__(Fft<L>::NB) void k_azimuth_fft(
         const unsigned code = codes[pad32(q)];
         const int kind = (int)(code >> 30), idx = (int)((code >> 15) & 0x7fffu), wq = (int)(code & 0x7fffu);
         short2 v = make_short2(0, 0);
-        if (kind == 1) v = sinc_q15w(rowl, idx, sinc + (wq >> 7) * 32);
+        if (kind == 1) v = SPLIT ? sinc_q15s(rowl, idx, sinc + (wq >> 7) * 32)
+                                 : sinc_q15w(rowl, idx, sinc + (wq >> 7) * 32);
         else if (kind == 2) v = lerp_q15(rowl, idx, wq, false);
         if (do_window) {
             const int cw = wrap16((hr * hamc[q]) >> 15);
@@ -708,7 +845,8 @@ int launch_fixed_range(pfa_plan *p, const void *sig, void *g1t, int ldp) {
     pfa_ctx *ctx = p->ctx;
     dim3 grid((p->m + RT - 1) / RT, p->grid / QT);
     pfa_timer_begin(ctx, PFA_TIMER_RANGE);
-    hipLaunchKernelGGL(k_range_resample, grid, dim3(256), 0, ctx->stream, (const short2 *)sig,
+    hipLaunchKernelGGL((p->flags & FIXED_SINC_SPLIT) ? k_range_resample<true> : k_range_resample<false>,
+                       grid, dim3(256), 0, ctx->stream, (const short2 *)sig,
                        p->m, p->n, p->grid, p->a32, p->f0, p->df, p->order, p->krp, p->sinc,
                        p->flags, (short2 *)g1t, ldp);
     pfa_timer_end(ctx, PFA_TIMER_RANGE);
@@ -725,18 +863,16 @@ int launch_fixed_azimuth_fft(pfa_plan *p, const void *g1t, int ldp, void *ya, in
     const int L = log2_exact(p->grid);
     pfa_timer_begin(ctx, PFA_TIMER_AZIMUTH);
 #define LAUNCH(LL)                                                                             \
-    case LL:                                                                                   \
-        if (do_fft)                                                                            \
-            hipLaunchKernelGGL((k_azimuth_fft<LL, true>), dim3(p->grid), dim3(Fft<LL>::NB), 0, \
-                               ctx->stream, (const short2 *)g1t, ldp, p->m, p->krp, p->tan_s,  \
-                               p->kcp, p->sinc, p->flags, p->hamr, p->hamc, (int)do_window,    \
-                               (short2 *)ya, ea, emax_a, tw);                                  \
-        else                                                                                   \
-            hipLaunchKernelGGL((k_azimuth_fft<LL, false>), dim3(p->grid), dim3(Fft<LL>::NB), 0,\
-                               ctx->stream, (const short2 *)g1t, ldp, p->m, p->krp, p->tan_s,  \
-                               p->kcp, p->sinc, p->flags, p->hamr, p->hamc, (int)do_window,    \
-                               (short2 *)ya, ea, emax_a, tw);                                  \
-        break;
+    case LL: {                                                                                 \
+        const bool sp = p->flags & FIXED_SINC_SPLIT;                                           \
+        auto kern = do_fft ? (sp ? k_azimuth_fft<LL, true, true> : k_azimuth_fft<LL, true, false>) \
+                           : (sp ? k_azimuth_fft<LL, false, true> : k_azimuth_fft<LL, false, false>); \
+        hipLaunchKernelGGL(kern, dim3(p->grid), dim3(Fft<LL>::NB), 0,                          \
+                           ctx->stream, (const short2 *)g1t, ldp, p->m, p->krp, p->tan_s,      \
+                           p->kcp, p->sinc, p->flags, p->hamr, p->hamc, (int)do_window,        \
+                           (short2 *)ya, ea, emax_a, tw);                                      \
+        break;                                                                                 \
+    }
     PFA_SWITCH_L(L, LAUNCH)
 #undef LAUNCH
     pfa_timer_end(ctx, PFA_TIMER_AZIMUTH);

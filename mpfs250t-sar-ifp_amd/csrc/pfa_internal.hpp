@@ -45,6 +45,10 @@ struct pfa_ctx {
 
 enum PlanKind { PLAN_FIXED = 1, PLAN_FLOAT = 2 };
 
+// Internal flag of a fixed plan (beside the PFA_FIXED_* flags): the sinc
+// table's half sums allow the one-dot-product-per-pair gather.
+constexpr int FIXED_SINC_SPLIT = 0x100;
+
 
 struct pfa_plan {
     PlanKind kind;
