@@ -232,6 +232,17 @@ int pfa_resample_kspace_f32(pfa_plan *plan, const float *sig_c64, const double *
                             float *g2_c64);
 /* focus (form_image_pfa.py:225-240) of a given g2 complex64 [m][n]. */
 int pfa_transform_f32(pfa_plan *plan, const float *g2_c64, void *out, int out_kind);
+/* quicklook (form_image_pfa.py:158-162): sig complex64 [m][n] times
+ * hamming2d (window != 0; np.hamming tapers, f32(hm[i] hn[k])), then
+ * transform2d (:144-147) -- fftshift(fft2(ifftshift(x))) for sgn < 0, ifft2
+ * for sgn > 0 -- with no padding: out [m][n] as out_kind.  Any m, n in
+ * 1..8192, or powers of two up to 16384 (other lengths run as Bluestein
+ * transforms on power-of-two LDS FFTs).  Host pointers (synchronous) or
+ * device pointers (_dev, enqueued on the context stream). */
+int pfa_quicklook_f32(pfa_ctx *ctx, const float *sig_c64, int m, int n, int sgn, int window,
+                      void *out, int out_kind);
+int pfa_quicklook_f32_dev(pfa_ctx *ctx, const float *sig_c64, int m, int n, int sgn,
+                          int window, void *out, int out_kind);
 /* The O(m n) part of the scene geometry (form_image_pfa.py:187-192,
  * sar_pipeline.prepare_tables :92-98): kr = (2 freq / c) pr[:, None] in FP64,
  * each operation rounded as numpy evaluates it, reduced per pulse to

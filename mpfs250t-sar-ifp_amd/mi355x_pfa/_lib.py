@@ -67,6 +67,8 @@ PROTOS = {
     "pfa_float_geometry": [_i, _i] + [_vp] * 13,
     "pfa_plan_float_pfa": [_vp, C.POINTER(_vp), _i, _i, _i] + [_vp] * 5 + [_i, _vp, _i, _vp],
     "pfa_pfa_f32": [_vp, _vp, _i, _i, _vp, _vp, _vp, _i, _i, _vp, _i, _vp],
+    "pfa_quicklook_f32": [_vp, _vp, _i, _i, _i, _i, _vp, _i],
+    "pfa_quicklook_f32_dev": [_vp, _vp, _i, _i, _i, _i, _vp, _i],
     "pfa_kr_extrema": [_vp, _vp, _vp, _i, _i, _vp, _vp],
     "pfa_kr_extrema_dev": [_vp, _vp, _vp, _i, _i, _vp, _vp],
     "pfa_hist_u16": [_vp, _vp, _i64, _vp],

@@ -231,23 +231,28 @@ def focus(g2, geo, sgn, device=0):
 
 
 def quicklook(sig, sgn, window=True, device=0):
-    """form_image_pfa.quicklook -> complex64 (m, n): window + one centred 2-D
-    FFT of the raw history (the polar grid taken as Cartesian), no padding.
-    Both sizes must be powers of two (pulses 64 ... 8192, samples 64 ...
-    16384); the reference's numpy FFT takes any size, and other sizes are
-    refused here."""
-    sig = np.ascontiguousarray(sig, np.complex64)
-    m, n = sig.shape
-    for s, top in ((m, 8192), (n, 16384)):
-        if s < 2 or s & (s - 1) or s > top:
-            raise PfaError(-1, f"quicklook sizes must be powers of two up to 8192 pulses x "
-                               f"16384 samples, got {m}x{n}")
-    dummy = {"pr": np.ones(m), "order": np.arange(m, dtype=np.int32), "tan_s": np.zeros(m),
-             "KR": np.ones(n), "KC": np.zeros(m), "geo": {}}
-    plan = FloatPlan(m, n, sgn, dummy, window=window, device=device)
-    img = plan.transform(sig)
-    plan.close()
-    return img
+    """form_image_pfa.quicklook -> complex64 (m, n): window (hamming2d) + one
+    centred 2-D FFT of the raw history (the polar grid taken as Cartesian), no
+    padding, any shape: lengths up to 8192 on either axis, or powers of two up
+    to 16384 (the reference's numpy FFT takes any size).  sig: numpy array or
+    a CUDA complex64 tensor (the result stays on its side)."""
+    ctx = context(device)
+    if isinstance(sig, np.ndarray) or not hasattr(sig, "is_cuda"):
+        x = np.ascontiguousarray(sig, np.complex64)
+        m, n = x.shape
+        out = np.empty((m, n), np.complex64)
+        ctx.check(_lib.lib().pfa_quicklook_f32(ctx.handle, ptr(x), m, n, int(sgn), int(window),
+                                               ptr(out), OUT_C64), "quicklook")
+        return out
+    import torch
+    x = sig.contiguous()
+    m, n = x.shape
+    out = torch.empty((m, n), dtype=torch.complex64, device=x.device)
+    torch.cuda.synchronize()
+    ctx.check(_lib.lib().pfa_quicklook_f32_dev(ctx.handle, ptr(x), m, n, int(sgn), int(window),
+                                               ptr(out), OUT_C64), "quicklook")
+    ctx.synchronize()
+    return out
 
 
 def pfa(sig, freq, ax, ay, sgn, window=True, interp="linear", device=0, fft="fp32"):

@@ -117,23 +117,51 @@ def test_compact_frequencies_equal_full_array(gp):
 
 
 @pytest.mark.parametrize("m,n,sgn,window", [(256, 512, -1, True), (1024, 1024, 1, True),
-                                            (512, 256, -1, False), (128, 16384, -1, True)])
+                                            (512, 256, -1, False), (128, 16384, -1, True),
+                                            (5634, 4319, -1, True), (4319, 5634, 1, True),
+                                            (33, 47, -1, True), (1, 7, 1, False), (7, 1, -1, True),
+                                            (4099, 64, -1, False), (100, 8192, 1, True),
+                                            (8191, 300, -1, True), (8192, 257, 1, False)])
 def test_quicklook_matches_oracle(gp, m, n, sgn, window):
-    """form_image_pfa.quicklook (window + one centred 2-D FFT of the raw history)."""
-    sc, freq, ax, ay = _scene(m, n, seed=m + n)
-    img = gp.quicklook(sc.sig, sgn, window=window)
-    ref = rfl.quicklook(sc.sig, sgn, window=window)
+    """form_image_pfa.quicklook (window + one centred 2-D FFT of the raw history,
+    no padding) at any shape, the board scene's 5634 x 4319 among them; odd
+    lengths take both ifftshift / fftshift conventions.  Against the oracle's
+    numpy (complex64 FFT) within 1e-5 of the peak with the same top peaks, and
+    against a complex128 transform of the same windowed history."""
+    import torch
+    from mi355x_pfa import synth
+    if min(m, n) >= 16:
+        sc = synth.make_scene(synth.SceneSpec(m, n, targets=4, seed=m + n,
+                                              scene_half=0.2 * min(m, n)), device="cuda")
+        sig = sc.sig.cpu().numpy()
+    else:
+        rs = np.random.RandomState(m * n)
+        sig = (rs.standard_normal((m, n)) + 1j * rs.standard_normal((m, n))).astype(np.complex64)
+    print(f"quicklook {m}x{n} sgn={sgn} window={window}", flush=True)
+    img = gp.quicklook(sig, sgn, window=window)
+    ref = rfl.quicklook(sig, sgn, window=window)
     assert img.shape == (m, n)
     assert pn_rms(img, ref) <= TOL
-    assert top_peaks(img, 4) == top_peaks(ref, 4)
+    x = sig.astype(np.complex128)
+    if window:
+        x = x * np.outer(np.hamming(m), np.hamming(n)).astype(np.float32)
+    fwd = np.fft.fft2 if sgn < 0 else np.fft.ifft2
+    ref64 = np.fft.fftshift(fwd(np.fft.ifftshift(x)))
+    err = pn_rms(img, ref64)
+    print(f"pn-RMS vs complex128 {err:.2e}", flush=True)
+    assert err <= TOL
+    if min(m, n) >= 16:
+        assert top_peaks(img, 4) == top_peaks(ref, 4)
+    dev = gp.quicklook(torch.from_numpy(sig).cuda(), sgn, window=window).cpu().numpy()
+    assert np.array_equal(dev, img)
 
 
 def test_quicklook_refuses_other_sizes(gp):
     from mi355x_pfa._lib import PfaError
-    with pytest.raises(PfaError, match="powers of two"):
-        gp.quicklook(np.zeros((300, 256), np.complex64), -1)
-    with pytest.raises(PfaError, match="powers of two"):
-        gp.quicklook(np.zeros((16384, 256), np.complex64), -1)
+    with pytest.raises(PfaError, match="1..8192"):
+        gp.quicklook(np.zeros((8193, 64), np.complex64), -1)
+    with pytest.raises(PfaError, match="1..8192"):
+        gp.quicklook(np.zeros((64, 32768), np.complex64), -1)
 
 
 def test_padded_size_limits(gp):
