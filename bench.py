@@ -65,6 +65,21 @@ def algorithmic_bytes_float(m, n, m2, n2, s_in=8, s_mid=8, s_a=8, s_out=4):
             "fft_b_detect": s_a * n * m2 + s_out * m2 * n2}
 
 
+def algorithmic_bytes_fixed(m, n, G):
+    """Compulsory HBM bytes per kernel of the fixed frame (int16 I/Q = 4 bytes,
+    uint16 image): the range pass reads the codes and writes the G corner-turned
+    rows, the azimuth pass + FFT-A reads them and writes A, FFT-B reads A and
+    writes B, detect reads B and writes the image (the B4 staging)."""
+    return {"range": 4 * m * n + 4 * m * G, "azimuth_fft": 4 * m * G + 4 * G * G,
+            "fft_b": 8 * G * G, "detect": 4 * G * G + 2 * G * G}
+
+
+def fixed_b3(m, n, G):
+    """The B3 staging of the same frame (FFT-B with detect fused), the
+    cross-design figure."""
+    return 4 * m * n + 8 * m * G + 8 * G * G + 2 * G * G
+
+
 def time_float(plan, sig, out, steps, warmup, torch):
     from mi355x_pfa import _lib
     ctx = plan.ctx
@@ -460,6 +475,13 @@ def main():
             traffic = json.loads(tfile.read_text()).get(dom)
         except Exception:
             traffic = None
+    prof_us = None                    # the same kernel's average under rocprofv3 --kernel-trace
+    pfile = REPO / "profiles" / f"kernel_trace_{a.config.lower()}.json"
+    if pfile.exists() and a.fft == "fp32" and a.interp == "sinc32":
+        try:
+            prof_us = json.loads(pfile.read_text())["float_fp32"].get(dom)
+        except Exception:
+            prof_us = None
     res = {
         "metric": METRIC, "value": round(wall, 4), "unit": "ms/frame",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(wall, 4),
@@ -473,7 +495,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1),
                      "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM, 4), "traffic": traffic,
-                     "kernel": dom, "bytes_per_launch": bytes_k[dom]},
+                     "kernel": dom, "bytes_per_launch": bytes_k[dom],
+                     "time_source": "hipEvents on the library's stream, mean over the timed frames",
+                     "rocprof_avg_us": prof_us,
+                     "frac_at_rocprof_avg": (round(bytes_k[dom] / (prof_us * 1e-6) / PEAK_HBM, 4)
+                                             if prof_us else None)},
         "frame_device_ms": round(frame_dev, 4),
         "frame_hbm_gbps_B3": round(B3 / (frame_dev * 1e-3) / 1e9, 1),
         "frame_roofline_frac_B3": round(B3 / (frame_dev * 1e-3) / PEAK_HBM, 4),
@@ -530,9 +556,19 @@ def main():
             y.copy_(x)
             hip.hipMemcpyAsync(y.data_ptr(), x.data_ptr(), 512 << 20, 3,
                                torch.cuda.current_stream().cuda_stream)
+        # read-mostly kernels of known size: torch's sum of the 512 MiB, and the
+        # library's uint8 stretch of 128 Mi float32 values (512 MiB read, 128 MiB
+        # written, 16-byte loads like the focus kernels)
+        from mi355x_pfa import _lib as L
+        xf = x.view(torch.float32)
+        u8 = torch.empty(xf.numel(), dtype=torch.uint8, device="cuda")
+        for _ in range(3):
+            x.sum()
+            ctx.check(L.lib().pfa_stretch_u8_dev(ctx.handle, xf.data_ptr(), xf.numel(), 1.0,
+                                                 u8.data_ptr()), "stretch")
         torch.cuda.synchronize()
         res["copy_calibration_bytes"] = 512 << 20
-        del x, y
+        del x, y, xf, u8
     forms = [f for f in a.forms.split(",") if f]
     if forms:
         print("[bench] FFT forms vs FP64", file=sys.stderr, flush=True)
@@ -545,8 +581,21 @@ def main():
         print("[bench] fixed-point frames", file=sys.stderr, flush=True)
         res["products"]["_with_cpu"] = not a.no_cpu
         fwall, fk = time_fixed(sc, ax, ay, geom, max(3, a.steps // 2), 2, torch, res["products"])
+        fb = algorithmic_bytes_fixed(m, n, 8192)
+        fdev = sum(fk.values())
+        fdom = max(fk, key=fk.get)
         res["fixed"] = {"ms_per_frame": round(fwall, 3),
                         "kernels_ms": {k: round(v, 4) for k, v in fk.items()},
+                        "kernels_gbps": {k: round(fb[k] / (fk[k] * 1e-3) / 1e9, 1) for k in fk},
+                        "roofline": {"bound": "hbm", "kernel": fdom,
+                                     "achieved": round(fb[fdom] / (fk[fdom] * 1e-3) / 1e9, 1),
+                                     "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                                     "frac": round(fb[fdom] / (fk[fdom] * 1e-3) / PEAK_HBM, 4),
+                                     "bytes_per_launch": fb[fdom]},
+                        "frame_device_ms": round(fdev, 4),
+                        "frame_hbm_gbps_B4": round(sum(fb.values()) / (fdev * 1e-3) / 1e9, 1),
+                        "frame_roofline_frac_B4": round(sum(fb.values()) / (fdev * 1e-3) / PEAK_HBM, 4),
+                        "frame_roofline_frac_B3": round(fixed_b3(m, n, 8192) / (fdev * 1e-3) / PEAK_HBM, 4),
                         "speedup_vs_fpga_silicon": round(FPGA_FRAME_MS / fwall, 1)}
     if not a.no_cpu:
         print("[bench] cpu baseline (numpy oracle: the whole frame, np.interp)", file=sys.stderr,

@@ -633,7 +633,9 @@ extern "C" int pfa_plan_float(pfa_ctx *ctx, pfa_plan **out, int m, int n, int sg
     const int m2 = std::max(64, pow2_ceil(m)), n2 = std::max(64, pow2_ceil(n));
     // the cross line (m2) is transformed in LDS with its resample; the range
     // transform (n2) is one line per workgroup and reaches 16384 (FP32 form)
-    if (m2 > 8192) return pfa_fail(ctx, PFA_ERR_DIMS, "plan_float: padded pulse (cross) size above 8192");
+    if (m2 > 16384) return pfa_fail(ctx, PFA_ERR_DIMS, "plan_float: padded pulse (cross) size above 16384");
+    if (m2 > 8192 && (flags & (PFA_FLOAT_FFT_BF16 | PFA_FLOAT_FFT_FP16)))
+        return pfa_fail(ctx, PFA_ERR_DIMS, "plan_float: matrix-core FFT sizes are 256, 512, 1024, 4096 and 8192");
     if (n2 > 16384) return pfa_fail(ctx, PFA_ERR_DIMS, "plan_float: padded sample (range) size above 16384");
     if (pow2_ceil(m) != m2 || pow2_ceil(n) != n2)
         return pfa_fail(ctx, PFA_ERR_DIMS, "plan_float: m and n must be at least 33 (padded size >= 64)");
@@ -671,6 +673,18 @@ extern "C" int pfa_plan_float(pfa_ctx *ctx, pfa_plan **out, int m, int n, int sg
     if (ok && f0) ok = (p->f0d = upload(p, f0, m, &st)) && (p->dfd = upload(p, df, m, &st));
     if (ok && ham_c) ok = (p->hm = upload(p, ham_c, m, &st)) && (p->hn = upload(p, ham_r, n, &st));
     if (ok && sinc_tab) ok = (p->sinc = upload(p, sinc_tab, 256 * 32, &st)) != nullptr;
+    if (ok && sinc_tab && m2 > 8192) {        // 16384-point cross lines read packed taps from global memory:
+        constexpr int TPR = 20, TPH = TPR * 256;   // rows (2r + h, 2r + h + 1) of the taps padded by 4
+        std::vector<uint32_t> tp(2 * TPH);         // zero rows either side, phase index ph ^ (ph >> 3)
+        for (int i = 0; i < 2 * TPH; ++i) {
+            const int h = i / TPH, r = (i / 256) % TPR, ph = i & 255;
+            const int ra = 2 * r + h - 4, rb = ra + 1;
+            const int a = (ra >= 0 && ra < 32) ? sinc_tab[ph * 32 + ra] : 0;
+            const int b = (rb >= 0 && rb < 32) ? sinc_tab[ph * 32 + rb] : 0;
+            tp[h * TPH + r * 256 + (ph ^ (ph >> 3))] = ((uint32_t)a & 0xffffu) | ((uint32_t)b << 16);
+        }
+        ok = (p->taps2 = upload(p, tp.data(), tp.size(), &st)) != nullptr;
+    }
     if (ok) {                                 // tan bins: 2 per pulse over [tan_s[0], tan_s[m-1]]
         const double lo = tan_s[0], hi = tan_s[m - 1];
         int bins = 2 * m;

@@ -479,21 +479,24 @@ __global__ __launch_bounds__(256) void k_range_resample(
         row_s[pl] = p < m ? i : -1;
     }
     __syncthreads();
-    {                                              // all loads in flight, then the stores
-        constexpr int NL = RT * SW / 256;
-        static_assert(NL * 256 == RT * SW, "strips split evenly over the threads");
-        short2 v[NL];
+    {                                              // loads in flight in two batches, then the stores
+        constexpr int NL = RT * SW / 256, NH = NL / 2;
+        static_assert(NL * 256 == RT * SW && NH * 2 == NL, "strips split evenly over the threads");
 #pragma unroll
-        for (int k = 0; k < NL; ++k) {
-            const int f = t + 256 * k, pp = f / SW, u = f % SW;
-            const int r = row_s[pp], src = min(lo_s[pp] + u, n - 1);
-            v[k] = sig[(size_t)max(r, 0) * n + src];      // clamped: always in bounds
-        }
+        for (int h = 0; h < 2; ++h) {
+            short2 v[NH];
 #pragma unroll
-        for (int k = 0; k < NL; ++k) {
-            const int f = t + 256 * k, pp = f / SW, u = f % SW;
-            const bool ok = row_s[pp] >= 0 && lo_s[pp] + u < n;
-            strip[pp * SWP + u] = ok ? v[k] : make_short2(0, 0);
+            for (int k = 0; k < NH; ++k) {
+                const int f = t + 256 * (h * NH + k), pp = f / SW, u = f % SW;
+                const int r = row_s[pp], src = min(lo_s[pp] + u, n - 1);
+                v[k] = sig[(size_t)max(r, 0) * n + src];  // clamped: always in bounds
+            }
+#pragma unroll
+            for (int k = 0; k < NH; ++k) {
+                const int f = t + 256 * (h * NH + k), pp = f / SW, u = f % SW;
+                const bool ok = row_s[pp] >= 0 && lo_s[pp] + u < n;
+                strip[pp * SWP + u] = ok ? v[k] : make_short2(0, 0);
+            }
         }
     }
     __syncthreads();
@@ -503,7 +506,7 @@ __global__ __launch_bounds__(256) void k_range_resample(
         const int base = lo_s[pl];
         const short2 *sp = strip + pl * SWP;
         const short2 *row = sig + (size_t)i * n;
-#pragma unroll 4
+#pragma unroll
         for (int u = 0; u < PER; ++u) {
             const int cd = code[u];
             short2 v = make_short2(0, 0);
@@ -561,6 +564,8 @@ __global__ __launch_bounds__(Fft<L>::NB) void k_azimuth_fft(
     using F = Fft<L>;
     constexpr int G = F::N;
     constexpr int NB = F::NB;
+    constexpr int PER = G / NB > 0 ? G / NB : 1;
+    static_assert(PER * NB == G, "the threads' outputs cover the line");
     __shared__ short2 data[F::DATA];
     __shared__ float srcv[G];            // sources for the bracket search, then the row, then twiddles
     __shared__ int red[2 * F::NW];
@@ -569,36 +574,41 @@ __global__ __launch_bounds__(Fft<L>::NB) void k_azimuth_fft(
     const int j = blockIdx.x;
     const int t = threadIdx.x;
     const float kr = krp[j];
-    constexpr int PER = G / NB > 0 ? G / NB : 1;
-    {                                    // positions: all loads in flight before the first product
-        constexpr int CH = PER < 8 ? PER : 8;
-#pragma unroll 1
-        for (int i0 = 0; i0 < PER; i0 += CH) {
-            float ts[CH];
+    const short2 *row = g1t + (size_t)j * ldp;
+    // everything the line reads from global memory is requested up front
+    // (clamped slots, masked when used), so the loads overlap one another and
+    // the bracket phase instead of waiting one by one
+    float ts[PER], kq[PER];
+    short2 rw[PER];
 #pragma unroll
-            for (int i = 0; i < CH; ++i) ts[i] = tan_s[min(t + (i0 + i) * NB, m - 1)];
-#pragma unroll
-            for (int i = 0; i < CH; ++i)
-                if (t + (i0 + i) * NB < m) srcv[t + (i0 + i) * NB] = __fmul_rn(kr, ts[i]);
-        }
+    for (int i = 0; i < PER; ++i) {
+        ts[i] = tan_s[min(t + i * NB, m - 1)];
+        kq[i] = kcp[t * PER + i];
+        rw[i] = row[min(t + i * NB, m - 1)];
     }
+#pragma unroll
+    for (int i = 0; i < PER; ++i)
+        if (t + i * NB < m) srcv[t + i * NB] = __fmul_rn(kr, ts[i]);
     __syncthreads();
     const bool asc = srcv[m - 1] >= srcv[0];
     auto sa = [&](int u) { return srcv[asc ? u : m - 1 - u]; };
     const bool use_sinc = flags & PFA_FIXED_AZ_SINC;
     // (A) brackets and weights: c = #{u : sa(u) < Q}, with xl = sa(c - 1) and
-    // xh = sa(c) kept from one output to the next (KCp is non-decreasing)
-    int c = -1;
-    float xl = -INFINITY, xh = INFINITY;
-    for (int u = 0; u < PER; ++u) {
-        const int q = t * PER + u;
-        if (q >= G) break;
-        const float Q = kcp[q];
+    // xh = sa(c) kept from one output to the next (KCp is non-decreasing).  The
+    // first output starts from the linear guess between the end positions.
+    int c;
+    float xl, xh;
+    {
+        const float s0 = sa(0), s1 = sa(m - 1), Q = kq[0];
+        const float g = (s1 > s0) ? (Q - s0) / (s1 - s0) * (float)(m - 1) : 0.f;
+        c = (g == g) ? (int)fminf(fmaxf(g + 1.0f, 0.f), (float)m) : 0;
+        xl = c > 0 ? sa(c - 1) : -INFINITY;
+        xh = c < m ? sa(c) : INFINITY;
         int steps = 0;
-        if (c >= 0)
-            while (c < m && xh < Q && steps < 64) { ++c; ++steps; xl = xh; xh = c < m ? sa(c) : INFINITY; }
-        if (c < 0 || (c < m && xh < Q)) {        // first output, or far: bisect
-            int lo = c < 0 ? 0 : c, hi = m;
+        while (c < m && xh < Q && steps < 32) { ++c; ++steps; xl = xh; xh = c < m ? sa(c) : INFINITY; }
+        while (c > 0 && !(xl < Q) && steps < 64) { --c; ++steps; xh = xl; xl = c > 0 ? sa(c - 1) : -INFINITY; }
+        if (!((c == 0 || xl < Q) && (c == m || !(xh < Q)))) {   // far from the guess: bisect
+            int lo = 0, hi = m;
             while (lo < hi) {
                 const int mid = (lo + hi) >> 1;
                 if (sa(mid) < Q) lo = mid + 1; else hi = mid;
@@ -606,6 +616,25 @@ __global__ __launch_bounds__(Fft<L>::NB) void k_azimuth_fft(
             c = lo;
             xl = c > 0 ? sa(c - 1) : -INFINITY;
             xh = c < m ? sa(c) : INFINITY;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int q = t * PER + u;
+        const float Q = kq[u];
+        if (u > 0) {
+            int steps = 0;
+            while (c < m && xh < Q && steps < 64) { ++c; ++steps; xl = xh; xh = c < m ? sa(c) : INFINITY; }
+            if (c < m && xh < Q) {               // far: bisect above c
+                int lo = c, hi = m;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (sa(mid) < Q) lo = mid + 1; else hi = mid;
+                }
+                c = lo;
+                xl = c > 0 ? sa(c - 1) : -INFINITY;
+                xh = c < m ? sa(c) : INFINITY;
+            }
         }
         const int k = c - 1;
         unsigned code = 0;                                   // kind 0: zero output
@@ -618,17 +647,25 @@ __global__ __launch_bounds__(Fft<L>::NB) void k_azimuth_fft(
         }
         codes[pad32(q)] = code;
     }
+    int hc[PER];                         // the window's cross tapers, in flight during the row store
+    const int hr = hamr[j];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) hc[u] = do_window ? hamc[t + u * NB] : 0;
     __syncthreads();                     // positions dead: the row takes over
-    {
-        const short2 *row = g1t + (size_t)j * ldp;
-        for (int p = t; p < m; p += NB) rowl[p] = row[p];
-    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i)
+        if (t + i * NB < m) rowl[t + i * NB] = rw[i];
     __syncthreads();
     // (B) gathers and window
-    const int hr = hamr[j];
+    constexpr int TWP = DO_FFT ? (G / 2 + NB - 1) / NB : 1;
+    short2 twr[TWP];                     // the FFT's twiddles, in flight during the gathers
+    if constexpr (DO_FFT) {
+#pragma unroll
+        for (int i = 0; i < TWP; ++i) twr[i] = twg[min(t + i * NB, G / 2 - 1)];
+    }
+#pragma unroll
     for (int u = 0; u < PER; ++u) {
         const int q = t + u * NB;
-        if (q >= G) break;
         const unsigned code = codes[pad32(q)];
         const int kind = (int)(code >> 30), idx = (int)((code >> 15) & 0x7fffu), wq = (int)(code & 0x7fffu);
         short2 v = make_short2(0, 0);
@@ -636,7 +673,7 @@ __global__ __launch_bounds__(Fft<L>::NB) void k_azimuth_fft(
                                  : sinc_q15w(rowl, idx, sinc + (wq >> 7) * 32);
         else if (kind == 2) v = lerp_q15(rowl, idx, wq, false);
         if (do_window) {
-            const int cw = wrap16((hr * hamc[q]) >> 15);
+            const int cw = wrap16((hr * hc[u]) >> 15);
             v = make_short2((short)wrap16((v.x * cw) >> 15), (short)wrap16((v.y * cw) >> 15));
         }
         if (DO_FFT) data[pad32(q)] = v;
@@ -645,7 +682,9 @@ __global__ __launch_bounds__(Fft<L>::NB) void k_azimuth_fft(
     if constexpr (DO_FFT) {
         __syncthreads();                 // the row is dead: the twiddles take over
         short2 *tw = reinterpret_cast<short2 *>(srcv);
-        load_twiddles<L>(tw, twg);
+#pragma unroll
+        for (int i = 0; i < TWP; ++i)
+            if (t + i * NB < G / 2) tw[t + i * NB] = twr[i];
         __syncthreads();
         int xr[F::E], xi[F::E], e;
         F::run(data, tw, red, xr, xi, e);

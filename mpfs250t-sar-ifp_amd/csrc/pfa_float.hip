@@ -34,6 +34,12 @@
         MACRO(6) MACRO(7) MACRO(8) MACRO(9) MACRO(10) MACRO(11) MACRO(12) MACRO(13) \
         default: return pfa_fail(ctx, PFA_ERR_DIMS, "padded size must be a power of two in 64..8192"); \
     }
+// the azimuth pass with its cross transform: 16384-point lines as well
+#define PFA_F32_SWITCH_A(L, MACRO) \
+    switch (L) {                   \
+        MACRO(6) MACRO(7) MACRO(8) MACRO(9) MACRO(10) MACRO(11) MACRO(12) MACRO(13) MACRO(14) \
+        default: return pfa_fail(ctx, PFA_ERR_DIMS, "padded cross size must be a power of two in 64..16384"); \
+    }
 // the range transform (FFT-B) also takes 16384-point lines
 #define PFA_F32_SWITCH_B(L, MACRO) \
     switch (L) {                   \
@@ -70,12 +76,14 @@ int launch_float_azimuth(pfa_plan *p, int mode, const void *g1t, int ldp, const 
     const bool sinc = p->flags & PFA_FLOAT_SINC32;
     const bool inv = p->sgn > 0;
     const int L = log2_exact(m2);
+    if (L >= 14 && sinc && !p->taps2) return pfa_fail(ctx, PFA_ERR_STATE, "azimuth: no global tap table");
     // persistent: one resident workgroup per CU, a multiple of 8 (one per XCD slot)
     const int grid_az = std::max(8, std::min((p->n + 7) / 8 * 8, ctx->num_cu / 8 * 8));
     pfa_timer_begin(ctx, PFA_TIMER_AZIMUTH);
     const TanBins tb{p->tan_bin, p->tb_lo, p->tb_inv_w, p->tb_bins};
 #define ARGS (const float2 *)g1t, ldp, p->m, p->n, grid_of(p->KR, p->kr_lin, p->n), p->tan_s64, tb, grid_of(p->KC, p->kc_lin, p->m), p->hm, p->hn, p->sinc, \
-             (const float2 *)g2in, (float2 *)out, tw, (unsigned *)(ctx->d_counters + 32), nullptr, nullptr, (const float2 *)rot, nullptr
+             (const float2 *)g2in, (float2 *)out, tw, (unsigned *)(ctx->d_counters + 32), nullptr, nullptr, (const float2 *)rot, nullptr, \
+             (const unsigned *)p->taps2
 #define K4(LL, MODE)                                                                                 \
     if (sinc && inv) hipLaunchKernelGGL((k_azimuth_f32<LL, MODE, true, true>), dim3(grid_az), dim3(az_nb<LL>()), 0, ctx->stream, ARGS); \
     else if (sinc) hipLaunchKernelGGL((k_azimuth_f32<LL, MODE, true, false>), dim3(grid_az), dim3(az_nb<LL>()), 0, ctx->stream, ARGS); \
@@ -93,7 +101,7 @@ int launch_float_azimuth(pfa_plan *p, int mode, const void *g1t, int ldp, const 
         else if (mode == 1) { K4(LL, 1) }   \
         else { K4(LL, 2) }                  \
         break;
-    PFA_F32_SWITCH(L, LAUNCH)
+    PFA_F32_SWITCH_A(L, LAUNCH)
 #undef LAUNCH
 #undef K4R
 #undef K4

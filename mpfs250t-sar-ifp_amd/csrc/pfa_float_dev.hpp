@@ -478,7 +478,7 @@ __device__ __forceinline__ int xcd_map(int b, int count) {
 // 8192-point line runs on 1024 threads (16 waves keep the CU busy while one
 // workgroup holds the LDS).
 template <int L>
-constexpr int az_emax() { return 8; }
+constexpr int az_emax() { return L >= 14 ? 16 : 8; }   // 16384 points: 16 a thread on 1024 threads
 // The line FFT runs radix-16 passes (half the threads active) -- fewer LDS round
 // trips than radix 8 on all 1024.
 constexpr int AZ_FFT_EMAX = 16;
@@ -519,8 +519,13 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
     const int16_t *__restrict__ sinc, const float2 *__restrict__ g2in,
     float2 *__restrict__ out, const float2 *__restrict__ twg, unsigned *__restrict__ nslow,
     unsigned *__restrict__ A16, int *__restrict__ expA, const float2 *__restrict__ rot,
-    const pfam::u16x8 *__restrict__ ops) {
+    const pfam::u16x8 *__restrict__ ops, const unsigned *__restrict__ tapg) {
     static_assert(FK == FK_F32 || MODE == 1, "matrix-core FFT only in the fused mode");
+    // 16384-point lines (pulse counts above 8192): the positions, row and line
+    // take 140 KB of LDS, so the packed taps are read from global memory
+    // (tapg, the same layout) and the window is formed as it is applied
+    constexpr bool BIG = L >= 14;
+    static_assert(!BIG || FK == FK_F32, "16384-point lines in the FP32 form only");
     constexpr bool BF = FK == FK_BF16 || FK == FK_BF16S;
     constexpr bool SPLIT = FK == FK_BF16S;
     constexpr int EM = az_emax<L>();
@@ -531,13 +536,13 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
     constexpr int PR = (M2 + NB - 1) / NB;             // row samples per thread
     constexpr bool RES = MODE != 2;                    // resampling modes
     __shared__ float2 reg[(M2 + M2 / 16 + 64 > Cf::PADN + 64) ? M2 + M2 / 16 + 64 : Cf::PADN + 64];  // positions | row | FFT line
-    __shared__ unsigned tp[(SINC && RES) ? TPWORDS : 1];
+    __shared__ unsigned tp[(SINC && RES && !BIG) ? TPWORDS : 1];
     __shared__ float2 twl[M2 / 64 + 64];
     __shared__ float wmax[NB / 64];
-    __shared__ float wl[RES ? G * 4 * NB : 1];         // this line's window weights, thread-private slots
+    __shared__ float wl[(RES && !BIG) ? G * 4 * NB : 1];   // this line's window weights, thread-private slots
     double *xs = reinterpret_cast<double *>(reg);
     const int t = threadIdx.x;
-    if (SINC && RES) load_taps2(tp, sinc);
+    if (SINC && RES && !BIG) load_taps2(tp, sinc);
     const pfa32::Tw tw = pfa32::load_tw<M2>(twl, twg);
     const int per_x = (n + 7) / 8;
     const int ranks = (gridDim.x + 7) / 8;
@@ -605,11 +610,13 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
             int gix[G];
 #pragma unroll
             for (int g = 0; g < G; ++g) gix[g] = tb.bin(__dmul_rn(KC(min((g * NB + tl) * 4, m - 1)), inv_krj));
-            double hmv[G * 4];
+            double hmv[BIG ? 1 : G * 4];
 #pragma unroll
             for (int g = 0; g < G; ++g) gix[g] = tb.idx[gix[g]];
+            if constexpr (!BIG) {
 #pragma unroll
-            for (int k = 0; k < G * 4; ++k) hmv[k] = hm ? hm[min(((k / 4) * NB + tl) * 4 + k % 4, m - 1)] : 1.0;
+                for (int k = 0; k < G * 4; ++k) hmv[k] = hm ? hm[min(((k / 4) * NB + tl) * 4 + k % 4, m - 1)] : 1.0;
+            }
             int code[G * 4];
             float frq[G * 4];
 #pragma unroll
@@ -622,7 +629,7 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
                     const int k = g * 4 + u;
                     code[k] = 0;
                     frq[k] = 0.f;
-                    wl[k * NB + t] = (float)__dmul_rn(hmv[k], hj);
+                    if constexpr (!BIG) wl[k * NB + t] = (float)__dmul_rn(hmv[k], hj);
                     if (q >= m) continue;
                     const double Q = KC(q);
                     if (u == 0) {
@@ -688,7 +695,8 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
                 }
                 if (fast) {
                     float2 w4[4];
-                    sinc4p<6, true>(reg, J0 - 15, tp, ph, d, w4);
+                    if constexpr (BIG) sinc4p<6, true>(reg, J0 - 15, tapg, ph, d, w4);
+                    else sinc4p<6, true>(reg, J0 - 15, tp, ph, d, w4);
 #pragma unroll
                     for (int u = 0; u < 4; ++u) v[g * 4 + u] = w4[u];
                 } else {
@@ -702,7 +710,8 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
                             if (c == m) {
                                 if (ph[u]) r = fa(m - 1);
                             } else if (SINC && c - 1 >= 15 && c - 1 <= m - 17) {
-                                r = sinc1p(fa, c - 1, ph[u], tp);
+                                if constexpr (BIG) r = sinc1p(fa, c - 1, ph[u], tapg);
+                                else r = sinc1p(fa, c - 1, ph[u], tp);
                             } else {
                                 const float2 f0 = fa(c - 1), f1 = fa(c);
                                 const float fr = frq[k];
@@ -722,7 +731,13 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
             if (hm) {
 #pragma unroll
                 for (int k = 0; k < G * 4; ++k) {
-                    const float w = wl[k * NB + t];
+                    float w;
+                    if constexpr (BIG) {
+                        const int q = ((k / 4) * NB + tl) * 4 + k % 4;
+                        w = q < m ? (float)__dmul_rn(hm[q], hj) : 0.f;
+                    } else {
+                        w = wl[k * NB + t];
+                    }
                     v[k] = make_float2(v[k].x * w, v[k].y * w);
                 }
             }

@@ -71,6 +71,7 @@ struct pfa_plan {
     // bracket guesses of the azimuth pass: tan_bin[b] = #{p : tan_s[p] < tb_lo + b / tb_inv_w},
     // b = 0..tb_bins (a starting point only; the kernels fix brackets up exactly)
     int32_t *tan_bin = nullptr;
+    unsigned *taps2 = nullptr;            // packed Q15 tap pairs in global memory (16384-point lines)
     bool af_ready = false;                // autofocus checkpoint (range pass) computed
     void *af_g1t = nullptr;               // that checkpoint: the plan's own corner-turned range pass
     int tb_bins = 0;

@@ -160,7 +160,7 @@ int launch_float_azimuth_mfma(pfa_plan *p, const void *g1t, int ldp, unsigned *A
     pfa_timer_begin(ctx, PFA_TIMER_AZIMUTH);
 #define ARGS (const float2 *)g1t, ldp, p->m, p->n, grid_of(p->KR, p->kr_lin, p->n), p->tan_s64, tb, \
              grid_of(p->KC, p->kc_lin, p->m), p->hm, p->hn, p->sinc, nullptr, nullptr, tw,          \
-             (unsigned *)(ctx->d_counters + 32), A16, expA, nullptr, ops
+             (unsigned *)(ctx->d_counters + 32), A16, expA, nullptr, ops, nullptr
 #define K(LL, S, I, F) hipLaunchKernelGGL((k_azimuth_f32<LL, 1, S, I, F>), dim3(grid_az), dim3(az_nb<LL>()), 0, ctx->stream, ARGS)
 #define KF(LL, F)                                 \
     if (sinc && inv) K(LL, true, true, F);        \

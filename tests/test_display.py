@@ -82,3 +82,25 @@ def test_geotiff_u8_closed_form():
     expect = np.minimum(np.floor(amp.astype(np.float64) / 0.997 * 255), 255)
     assert np.max(np.abs(data.astype(int) - expect)) <= 1
     assert np.all(data[amp >= hi] == 255)
+
+
+def test_normal_score_lut_without_scipy_follows_the_reference_fallback(monkeypatch):
+    """sar_display.py:46-55 inverts the normal CDF with scipy's erfinv and,
+    where scipy is missing, by np.interp on a 200001-point erf table over
+    [-8, 8].  The product keeps that fallback for parity; with scipy.special
+    made unimportable it runs, and its LUT stays within one level of the
+    erfinv LUT (the table's spacing, 8e-5 in z, is far below one level of
+    127 / 3 per unit z)."""
+    import sys
+    from mi355x_pfa import display
+    rs = np.random.RandomState(4)
+    img = np.minimum(rs.rayleigh(900.0, 200000), 65535).astype(np.uint16)
+    img[:500] = 0
+    hist = np.bincount(img, minlength=65536)
+    lut_scipy, lo, blank = display.normal_score_lut(hist)
+    monkeypatch.setitem(sys.modules, "scipy.special", None)
+    lut_fb, lo2, blank2 = display.normal_score_lut(hist)
+    assert (lo, blank) == (lo2, blank2) and lut_fb.shape == lut_scipy.shape
+    d = np.abs(lut_fb.astype(int) - lut_scipy.astype(int))
+    assert d.max() <= 1 and (d > 0).mean() < 0.01
+    assert lut_fb[0 - lo] == 0 if lo == 0 else True

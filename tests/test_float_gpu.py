@@ -165,21 +165,45 @@ def test_quicklook_refuses_other_sizes(gp):
 
 
 def test_padded_size_limits(gp):
-    """Padded range (sample) size up to 16384, padded cross (pulse) size up to
-    8192; beyond either the plan is refused with a dimension error."""
+    """Padded range (sample) and cross (pulse) sizes up to 16384 in the FP32
+    form; beyond either the plan is refused with a dimension error, and the
+    matrix-core forms stop at 8192."""
     from mi355x_pfa._lib import PfaError
 
-    def plan(m, n):
+    def plan(m, n, fft="fp32"):
         geom = {"pr": np.ones(m), "order": np.arange(m, dtype=np.int32), "tan_s": np.zeros(m),
                 "KR": np.linspace(1.0, 2.0, n), "KC": np.linspace(-1.0, 1.0, m), "geo": {}}
-        return gp.FloatPlan(m, n, -1, geom, f0=np.ones(m), df=np.ones(m))
+        return gp.FloatPlan(m, n, -1, geom, f0=np.ones(m), df=np.ones(m), fft=fft)
     p = plan(100, 16384)
     assert (p.m2, p.n2) == (128, 16384)
+    p.close()
+    p = plan(16384, 100)
+    assert (p.m2, p.n2) == (16384, 128)
     p.close()
     with pytest.raises(PfaError, match="range"):
         plan(100, 16385)
     with pytest.raises(PfaError, match="cross"):
-        plan(8193, 100)
+        plan(16385, 100)
+    with pytest.raises(PfaError, match="matrix-core"):
+        plan(9000, 256, fft="bf16")
+
+
+@pytest.mark.parametrize("m,n,sgn,interp", [(9000, 300, -1, "linear"), (12000, 200, 1, "linear"),
+                                            (8200, 256, -1, "sinc32"), (16384, 128, -1, "sinc32")])
+def test_cross_sizes_above_8192(gp, m, n, sgn, interp):
+    """Scenes of more than 8192 pulses (padded cross size 16384,
+    form_image_pfa.py:225-240): the np.interp focus against the oracle's numpy
+    pfa, SINC32 against the FP64 oracle, within 1e-5 with the same peaks; the
+    windowed k-space of resample_kspace elementwise as well."""
+    print(f"cross size above 8192: {m}x{n} sgn={sgn} {interp}", flush=True)
+    sc, freq, ax, ay = _scene(m, n, seed=m % 1000 + n, targets=8)
+    img, geo = gp.pfa(sc.sig, freq, ax, ay, sgn, interp=interp)
+    store = np.complex64 if interp == "linear" else np.complex128
+    ref, geo_r = rfl.pfa(sc.sig, freq, ax, ay, sgn, interp=interp, store=store)
+    assert img.shape == (16384, 512 if n > 256 else 256 if n > 128 else 128)
+    assert pn_rms(img, ref) <= TOL
+    assert top_peaks(img, 4) == top_peaks(ref, 4)
+    assert [geo["dc"], geo["dr"]] == [geo_r["dc"], geo_r["dr"]]
 
 
 @pytest.mark.parametrize("desc", [False, True])
