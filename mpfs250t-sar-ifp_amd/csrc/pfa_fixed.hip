@@ -251,7 +251,8 @@ __global__ void k_zero_i32(int *p, int count) {
 
 // fixedpoint.fit_scale's max|re|, max|im| over a complex64 array (float bits of
 // non-negative values order like unsigned integers).
-__global__ void k_absmax(const float *x, int64_t count, unsigned *maxbits) {
+// Scalar forms, for arrays not aligned for the vector forms below.
+__global__ void k_absmax1(const float *x, int64_t count, unsigned *maxbits) {
     unsigned m = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
          i += (int64_t)gridDim.x * blockDim.x)
@@ -260,9 +261,46 @@ __global__ void k_absmax(const float *x, int64_t count, unsigned *maxbits) {
     if ((threadIdx.x & 63) == 0) atomicMax(maxbits, m);
 }
 
-__global__ void k_quantize(const float *x, int64_t count, float lsb, int16_t *out) {
+__global__ void k_quantize1(const float *x, int64_t count, float lsb, int16_t *out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) out[i] = (int16_t)fminf(fmaxf(floorf(x[i] / lsb), -32768.f), 32767.f);
+}
+
+// 16-byte loads over the whole float4 part of the array; the count % 4 tail
+// values are taken by the first threads.
+__global__ void k_absmax(const float *x, int64_t count, unsigned *maxbits) {
+    unsigned m = 0;
+    const int64_t n4 = count / 4;
+    const float4 *x4 = reinterpret_cast<const float4 *>(x);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+        const float4 v = x4[i];
+        m = max(max(max(m, __float_as_uint(fabsf(v.x))), __float_as_uint(fabsf(v.y))),
+                max(__float_as_uint(fabsf(v.z)), __float_as_uint(fabsf(v.w))));
+    }
+    const int64_t tail = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tail < count - 4 * n4) m = max(m, __float_as_uint(fabsf(x[4 * n4 + tail])));
+    for (int off = 32; off >= 1; off >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(maxbits, m);
+}
+
+// x / 2^e is exact, so the quotient, floor and clamp give fit_scale's codes.
+__device__ __forceinline__ int16_t quant1(float v, float lsb) {
+    return (int16_t)fminf(fmaxf(floorf(v / lsb), -32768.f), 32767.f);
+}
+__global__ void k_quantize(const float *x, int64_t count, float lsb, int16_t *out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;   // one group of four
+    if (4 * i + 3 < count) {
+        const float4 v = reinterpret_cast<const float4 *>(x)[i];
+        short4 q;
+        q.x = quant1(v.x, lsb);
+        q.y = quant1(v.y, lsb);
+        q.z = quant1(v.z, lsb);
+        q.w = quant1(v.w, lsb);
+        reinterpret_cast<short4 *>(out)[i] = q;
+    } else {
+        for (int64_t k = 4 * i; k < count; ++k) out[k] = quant1(x[k], lsb);
+    }
 }
 
 // ------------------------------------------------------------------ resample helpers
@@ -854,9 +892,10 @@ int launch_quantize(pfa_ctx *ctx, const float *sig, int64_t count, int16_t *out,
     // count = number of float values (2 per complex sample)
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, ctx->stream, (int *)ctx->d_maxbits, 1);
     const int blocks = (int)std::min<int64_t>(4096, grid1(count, 256));
+    const bool vec = ((uintptr_t)sig & 15) == 0 && ((uintptr_t)out & 7) == 0;
     if (count > 0)
-        hipLaunchKernelGGL(k_absmax, dim3(blocks), dim3(256), 0, ctx->stream, sig, count,
-                           ctx->d_maxbits);
+        hipLaunchKernelGGL(vec ? k_absmax : k_absmax1, dim3(blocks), dim3(256), 0, ctx->stream, sig,
+                           count, ctx->d_maxbits);
     PFA_HIP(ctx, hipGetLastError());
     unsigned bits = 0;
     PFA_HIP(ctx, hipMemcpyAsync(&bits, ctx->d_maxbits, 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -873,8 +912,11 @@ int launch_quantize(pfa_ctx *ctx, const float *sig, int64_t count, int16_t *out,
     if (e < -126 || e > 127)
         return pfa_fail(ctx, PFA_ERR_GEOMETRY, "signal scale outside the float32 exponent range");
     *exp_out = e;
-    if (count > 0)
-        hipLaunchKernelGGL(k_quantize, dim3(grid1(count, 256)), dim3(256), 0, ctx->stream, sig,
+    if (count > 0 && vec)
+        hipLaunchKernelGGL(k_quantize, dim3(grid1((count + 3) / 4, 256)), dim3(256), 0, ctx->stream,
+                           sig, count, ldexpf(1.0f, e), out);
+    else if (count > 0)
+        hipLaunchKernelGGL(k_quantize1, dim3(grid1(count, 256)), dim3(256), 0, ctx->stream, sig,
                            count, ldexpf(1.0f, e), out);
     PFA_HIP(ctx, hipGetLastError());
     return PFA_OK;
