@@ -605,7 +605,11 @@ __global__ __launch_bounds__(Fft<L>::NB) void k_azimuth_fft(
     constexpr int PER = G / NB > 0 ? G / NB : 1;
     static_assert(PER * NB == G, "the threads' outputs cover the line");
     __shared__ short2 data[F::DATA];
-    __shared__ float srcv[G];            // sources for the bracket search, then the row, then twiddles
+    // sources for the bracket search, then the row, then twiddles.  The float32
+    // positions sit at si(p) = p + p / 16: each lane walks 16 consecutive outputs,
+    // so the lanes of a wave read positions about 16 floats apart, which unpadded
+    // would put a whole wave's reads on two of the 32 banks.
+    __shared__ float srcv[G + G / 16];
     __shared__ int red[2 * F::NW];
     unsigned *codes = reinterpret_cast<unsigned *>(data);
     short2 *rowl = reinterpret_cast<short2 *>(srcv);
@@ -624,12 +628,13 @@ __global__ __launch_bounds__(Fft<L>::NB) void k_azimuth_fft(
         kq[i] = kcp[t * PER + i];
         rw[i] = row[min(t + i * NB, m - 1)];
     }
+    auto si = [](int p) { return p + (p >> 4); };
 #pragma unroll
     for (int i = 0; i < PER; ++i)
-        if (t + i * NB < m) srcv[t + i * NB] = __fmul_rn(kr, ts[i]);
+        if (t + i * NB < m) srcv[si(t + i * NB)] = __fmul_rn(kr, ts[i]);
     __syncthreads();
-    const bool asc = srcv[m - 1] >= srcv[0];
-    auto sa = [&](int u) { return srcv[asc ? u : m - 1 - u]; };
+    const bool asc = srcv[si(m - 1)] >= srcv[0];
+    auto sa = [&](int u) { return srcv[si(asc ? u : m - 1 - u)]; };
     const bool use_sinc = flags & PFA_FIXED_AZ_SINC;
     // (A) brackets and weights: c = #{u : sa(u) < Q}, with xl = sa(c - 1) and
     // xh = sa(c) kept from one output to the next (KCp is non-decreasing).  The
