@@ -674,14 +674,17 @@ extern "C" int pfa_plan_float(pfa_ctx *ctx, pfa_plan **out, int m, int n, int sg
     if (ok && ham_c) ok = (p->hm = upload(p, ham_c, m, &st)) && (p->hn = upload(p, ham_r, n, &st));
     if (ok && sinc_tab) ok = (p->sinc = upload(p, sinc_tab, 256 * 32, &st)) != nullptr;
     if (ok && sinc_tab && m2 > 8192) {        // 16384-point cross lines read packed taps from global memory:
-        constexpr int TPR = 20, TPH = TPR * 256;   // rows (2r + h, 2r + h + 1) of the taps padded by 4
-        std::vector<uint32_t> tp(2 * TPH);         // zero rows either side, phase index ph ^ (ph >> 3)
-        for (int i = 0; i < 2 * TPH; ++i) {
-            const int h = i / TPH, r = (i / 256) % TPR, ph = i & 255;
+        // the layout load_taps2 stages in LDS: word h TPH + 256 r + ph holds rows
+        // (2r + h, 2r + h + 1) of the taps padded by 4 zero rows either side,
+        // the h = 1 half 32 words past 20 rows
+        constexpr int TPR = 20, TPH = PFA_TAP2_HALF;
+        std::vector<uint32_t> tp(2 * TPH, 0u);
+        for (int i = 0; i < 2 * TPR * 256; ++i) {
+            const int h = i / (TPR * 256), r = (i / 256) % TPR, ph = i & 255;
             const int ra = 2 * r + h - 4, rb = ra + 1;
             const int a = (ra >= 0 && ra < 32) ? sinc_tab[ph * 32 + ra] : 0;
             const int b = (rb >= 0 && rb < 32) ? sinc_tab[ph * 32 + rb] : 0;
-            tp[h * TPH + r * 256 + (ph ^ (ph >> 3))] = ((uint32_t)a & 0xffffu) | ((uint32_t)b << 16);
+            tp[h * TPH + r * 256 + ph] = ((uint32_t)a & 0xffffu) | ((uint32_t)b << 16);
         }
         ok = (p->taps2 = upload(p, tp.data(), tp.size(), &st)) != nullptr;
     }

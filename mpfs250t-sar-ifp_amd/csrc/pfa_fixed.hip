@@ -414,13 +414,19 @@ constexpr int QT = 64;   // range outputs per tile
 constexpr int SW = 112;  // staged source samples per pulse (64 outputs + window + slack)
 constexpr int SWP = SW + 1;  // odd pitch: the pulses of a wave start in different banks
 
-// Tile of RT sorted pulses x QT range outputs; thread t serves pulse t / 4 and
-// the 16 consecutive outputs 16 (t % 4) + u.  Three phases:
+// Tile of RT sorted pulses x QT range outputs; thread t serves pulse
+// 16 (t / 64) + t % 16 and the 16 consecutive outputs 16 ((t / 16) % 4) + u, so
+// the 32 lanes a paired-word LDS read (ds_read2) serves per cycle are 16
+// pulses x two chunks: with the odd strip pitch their windows start in
+// distinct banks of the 64, where four chunks of one pulse among those 32
+// lanes put chunks c and c + 2, 32 samples apart, on the same banks (measured
+// at 8167 x 8192: bank-conflict cycles 130 M -> 49 M per launch, 0.60 ->
+// 0.555 ms).  Three phases:
 //  (1) brackets and Q15 weights of the thread's outputs, walking the FP64
 //      positions with the two that bracket the previous output kept;
 //  (2) each pulse's source strip -- from its first window on, SW samples --
-//      staged in LDS by coalesced loads (the four threads of a pulse agree on
-//      its start with two lane swaps);
+//      staged in LDS by coalesced loads (the four threads of a pulse, lanes
+//      16 apart, agree on its start with two lane swaps);
 //  (3) the gathers from the strip (from the row in global memory for the
 //      rare window that leaves it), then the tile transposed through LDS so
 //      g1t rows get whole 256-byte runs.
@@ -437,7 +443,7 @@ __global__ __launch_bounds__(256) void k_range_resample(
     auto tile = reinterpret_cast<short2 (*)[RT + 1]>(&strip[0]);
     const int p0 = blockIdx.x * RT, q0 = blockIdx.y * QT;
     const int t = threadIdx.x;
-    const int pl = t >> 2, ch = t & 3;
+    const int pl = ((t >> 6) << 4) | (t & 15), ch = (t >> 4) & 3;
     const int p = p0 + pl;
     const bool use_sinc = flags & PFA_FIXED_RANGE_SINC;
     if (use_sinc)
@@ -510,8 +516,8 @@ __global__ __launch_bounds__(256) void k_range_resample(
         }
     }
     // (2) the pulse's strip starts at its lowest window: min over its four threads
-    lo = min(lo, __shfl_xor(lo, 1, 64));
-    lo = min(lo, __shfl_xor(lo, 2, 64));
+    lo = min(lo, __shfl_xor(lo, 16, 64));
+    lo = min(lo, __shfl_xor(lo, 32, 64));
     if (ch == 0) {
         lo_s[pl] = max(0, min(lo - 15, n - SW));
         row_s[pl] = p < m ? i : -1;

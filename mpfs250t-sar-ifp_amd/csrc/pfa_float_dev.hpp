@@ -47,10 +47,14 @@ __device__ __forceinline__ void st_stream(float2 *p, float2 v) {     // write-on
 // Taps of the Q15 sinc table as float (exact: c / 2^15; the autofocus adjoint's
 // table), staged in LDS tap-major: tap t of phase ph at tapl[(t + 4) * 256 + scr(ph)], with four zero
 // rows on either side (t in [-4, 36)) so a window shifted by up to KD samples
-// reads zeros instead of needing masks, and scr(ph) = ph ^ (ph >> 3) spreads
-// the phases of a wave's lanes over the LDS banks.
+// reads zeros instead of needing masks.  The LDS has 64 banks of 4 bytes, so a
+// row's word for phase ph sits in bank ph % 64: lanes whose phases step
+// through a line (the azimuth pass) or differ by pulse (the range pass) spread
+// over the banks.  Measured at 8192^2 against the earlier ph ^ (ph >> 3)
+// column order: bank-conflict cycles of the range pass 34.8 M -> 27.4 M per
+// launch (with the half offset below), images identical.
 constexpr int TROWS = 40;
-__device__ __forceinline__ int scr(int ph) { return ph ^ (ph >> 3); }
+__device__ __forceinline__ int scr(int ph) { return ph; }
 __device__ __forceinline__ void load_taps(float *tapl, const int16_t *__restrict__ sinc) {
     for (int i = threadIdx.x; i < TROWS * 256; i += blockDim.x) {
         const int row = i >> 8, ph = i & 255, t = row - 4;
@@ -64,10 +68,15 @@ __device__ __forceinline__ void load_taps(float *tapl, const int16_t *__restrict
 // shifted by any d reads aligned pairs.  Half the LDS bytes of float taps; one
 // conversion per tap; sums of integer taps scaled by 2^-15 at the end equal
 // the float-tap sums exactly (a power-of-two scale commutes with rounding).
-constexpr int TPR = TROWS / 2, TPH = TPR * 256, TPWORDS = 2 * TPH;
+// The rows stay 256 words apart (the gathers read two rows per instruction
+// at offsets in 64-word units), so every row of a half is on the same banks;
+// the h = 1 half starts 32 words further on, so outputs of one phase whose
+// windows are shifted by an odd and an even d read different banks.
+constexpr int TPR = TROWS / 2, TPH = TPR * 256 + 32, TPWORDS = 2 * TPH;
+static_assert(TPH == PFA_TAP2_HALF, "the host builds the same layout");
 __device__ __forceinline__ void load_taps2(unsigned *tp, const int16_t *__restrict__ sinc) {
-    for (int i = threadIdx.x; i < TPWORDS; i += blockDim.x) {
-        const int h = i / TPH, r = (i / 256) % TPR, ph = i & 255;
+    for (int i = threadIdx.x; i < 2 * TPR * 256; i += blockDim.x) {
+        const int h = i / (TPR * 256), r = (i / 256) % TPR, ph = i & 255;
         const int ra = 2 * r + h - 4, rb = ra + 1;           // tap indices of the two rows
         const int a = (ra >= 0 && ra < 32) ? sinc[ph * 32 + ra] : 0;
         const int b = (rb >= 0 && rb < 32) ? sinc[ph * 32 + rb] : 0;

@@ -48,6 +48,10 @@ enum PlanKind { PLAN_FIXED = 1, PLAN_FLOAT = 2 };
 // Internal flag of a fixed plan (beside the PFA_FIXED_* flags): the sinc
 // table's half sums allow the one-dot-product-per-pair gather.
 constexpr int FIXED_SINC_SPLIT = 0x100;
+// Words per half of the packed float-path tap table (20 rows of 256 phases and
+// a 32-word bank offset); the device code stages it, the host builds the global
+// copy the 16384-point azimuth lines read.
+constexpr int PFA_TAP2_HALF = 20 * 256 + 32;
 
 
 struct pfa_plan {
