@@ -574,6 +574,29 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
         for (int k = 0; k < PR; ++k) pre[k] = ld_stream(row + min(tp_ + k * NB, m - 1));
     };
     if (RES && active_at(0)) prefetch(line_of(0));
+    // the tan values of this thread's source positions are the same for every
+    // line: held in registers where they fit (the FP32 form of lines up to
+    // 8192 points), loaded per line otherwise
+    constexpr bool HOLD = RES && !BIG && FK == FK_F32;
+    double tsv[HOLD ? PR : 1];
+    if constexpr (HOLD) {
+#pragma unroll
+        for (int k = 0; k < PR; ++k) tsv[k] = tan_s[min(t + k * NB, m - 1)];
+    }
+    // tan-bin guesses of each group's first output, looked up during the
+    // previous line so the bracket phase starts without a global round trip
+    // (16384-point lines, short of registers, look them up at the line's start)
+    int gix[G];
+    auto guesses = [&](int jj, int (&gx)[G]) {
+        const double ik = 1.0 / KR(jj);
+        int to = t;                                    // opaque: the KC values are not held across lines
+        asm volatile("" : "+v"(to));
+#pragma unroll
+        for (int g = 0; g < G; ++g) gx[g] = tb.bin(__dmul_rn(KC(min((g * NB + to) * 4, m - 1)), ik));
+#pragma unroll
+        for (int g = 0; g < G; ++g) gx[g] = tb.idx[gx[g]];
+    };
+    if (RES && !BIG && active_at(0)) guesses(line_of(0), gix);
     for (int base = 0; base < per_x; base += ranks) {
         if (!active_at(base)) continue;               // uniform per workgroup
         const int j = line_of(base);
@@ -598,34 +621,30 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
             }
         } else {
             const double krj = KR(j);
-            const double inv_krj = 1.0 / krj;
             const double hj = hn ? hn[j] : 1.0;
-            // (1) FP64 source positions of this line (all loads in flight before
-            // the first product)
-            double tv[PR];
+            double hmv[BIG ? 1 : G * 4];              // the window's cross tapers, in flight
+            if constexpr (!BIG) {                     // while the positions are formed
 #pragma unroll
-            for (int k = 0; k < PR; ++k) tv[k] = tan_s[min(tl + k * NB, m - 1)];
+                for (int k = 0; k < G * 4; ++k) hmv[k] = hm ? hm[min(((k / 4) * NB + tl) * 4 + k % 4, m - 1)] : 1.0;
+            }
+            // (1) FP64 source positions of this line
+            double tv[HOLD ? 1 : PR];
+            if constexpr (!HOLD) {
+#pragma unroll
+                for (int k = 0; k < PR; ++k) tv[k] = tan_s[min(tl + k * NB, m - 1)];
+            }
             __syncthreads();                          // previous line fully consumed
 #pragma unroll
             for (int k = 0; k < PR; ++k)
-                if (tl + k * NB < m) xs[tl + k * NB] = __dmul_rn(krj, tv[k]);
+                if (tl + k * NB < m) xs[tl + k * NB] = __dmul_rn(krj, HOLD ? tsv[k] : tv[k]);
             __syncthreads();
+            int gixl[BIG ? G : 1];
+            if constexpr (BIG) guesses(j, gixl);
             // (2) brackets: c = #{p : xs[p] <= Q} (np.interp / searchsorted-right),
             // kept as (c << 8) | sinc phase, and the interval fraction.  The
             // first output of each group of four starts from the tan-bin guess;
             // the next three walk up from their predecessor's bracket (KC is
-            // non-decreasing).  Guesses and the window's taper values are
-            // loaded before any is used, so their global round trips overlap.
-            int gix[G];
-#pragma unroll
-            for (int g = 0; g < G; ++g) gix[g] = tb.bin(__dmul_rn(KC(min((g * NB + tl) * 4, m - 1)), inv_krj));
-            double hmv[BIG ? 1 : G * 4];
-#pragma unroll
-            for (int g = 0; g < G; ++g) gix[g] = tb.idx[gix[g]];
-            if constexpr (!BIG) {
-#pragma unroll
-                for (int k = 0; k < G * 4; ++k) hmv[k] = hm ? hm[min(((k / 4) * NB + tl) * 4 + k % 4, m - 1)] : 1.0;
-            }
+            // non-decreasing).
             int code[G * 4];
             float frq[G * 4];
 #pragma unroll
@@ -642,7 +661,8 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
                     if (q >= m) continue;
                     const double Q = KC(q);
                     if (u == 0) {
-                        c = gix[g];
+                        if constexpr (BIG) c = gixl[g];
+                        else c = gix[g];
                         xl = c > 0 ? xs[c - 1] : -INFINITY;
                         xh = c < m ? xs[c] : INFINITY;
                         if (xh <= Q) {                // the guess is low by one for ~1/4 of outputs
@@ -734,7 +754,10 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
             }
             {                                         // the next row, in flight during the FFT
                 const int nb2 = base + ranks;
-                if (nb2 < per_x && active_at(nb2)) prefetch(line_of(nb2));
+                if (nb2 < per_x && active_at(nb2)) {
+                    prefetch(line_of(nb2));
+                    if constexpr (!BIG) guesses(line_of(nb2), gix);
+                }
             }
             // window
             if (hm) {
