@@ -230,13 +230,18 @@ __global__ void k_window(const short2 *g2, int rows, int cols, const int32_t *ha
     out[i] = make_short2((short)wrap16((v.x * cw) >> 15), (short)wrap16((v.y * cw) >> 15));
 }
 
-// floor(sqrt(I^2 + Q^2)) of int16 codes, exact (the sum is at most 2^31).
+// floor(sqrt(I^2 + Q^2)) of int16 codes, exact (the sum s is at most 2^31).
+// The estimate is the hardware square root of float(s): float(s) is within 64
+// of s (exact below 2^24) and the root within 2^-22 relative, so the estimate
+// is within 0.02 of sqrt(s) and its truncation at most one off the integer
+// root; one step down and one step up fix it.  d <= 46341, so d^2 and
+// (d + 1)^2 are 24-bit products below 2^32.
 __device__ __forceinline__ uint16_t detect_one(short2 v) {
     const unsigned s = (unsigned)((int)v.x * v.x) + (unsigned)((int)v.y * v.y);
-    unsigned d = (unsigned)sqrtf((float)s);
-    while (d > 0 && (unsigned long long)d * d > s) --d;
-    while ((unsigned long long)(d + 1) * (d + 1) <= s) ++d;
-    return (uint16_t)min(d, 65535u);
+    unsigned d = (unsigned)__builtin_amdgcn_sqrtf((float)s);
+    d -= __umul24(d, d) > s ? 1u : 0u;
+    d += __umul24(d + 1, d + 1) <= s ? 1u : 0u;
+    return (uint16_t)d;
 }
 
 __global__ void k_detect(const short2 *z, int64_t count, uint16_t *out) {
