@@ -797,24 +797,33 @@ __global__ __launch_bounds__(Fft<L>::NB) void k_fft_b(const short2 *__restrict__
     }
 }
 
-// pass-B renorm + detect, transposing yb[cross k][range j] to out[range j][cross k].
+// pass-B renorm + detect, transposing yb[cross k][range j] to out[range j][cross k]
+// through a 64 x 64 LDS tile.  Each thread reads two neighbouring range bins of
+// a cross row (one 8-byte load) and writes two neighbouring cross bins of an
+// output row (one 4-byte store); the tile's odd word pitch (33) puts the rows a
+// wave touches on different banks.
 __global__ __launch_bounds__(256) void k_detect_t(const short2 *__restrict__ yb,
                                                   const int32_t *__restrict__ eb,
                                                   const int *__restrict__ emax_b, int G,
                                                   uint16_t *__restrict__ out) {
-    __shared__ uint16_t tile[64][65];
+    __shared__ uint16_t tile[64][66];
     const int k0 = blockIdx.y * 64, j0 = blockIdx.x * 64;
     const int emb = *emax_b;
-    for (int r = threadIdx.x; r < 64 * 64; r += 256) {
-        const int kk = r / 64, jj = r % 64;
-        const short2 v = yb[(size_t)(k0 + kk) * G + j0 + jj];
+#pragma unroll
+    for (int r = threadIdx.x; r < 64 * 32; r += 256) {
+        const int kk = r / 32, jj = (r % 32) * 2;
+        const uint2 w = *reinterpret_cast<const uint2 *>(yb + (size_t)(k0 + kk) * G + j0 + jj);
         const int sh = emb - eb[k0 + kk];
-        tile[jj][kk] = detect_one(make_short2((short)sat16(v.x >> sh), (short)sat16(v.y >> sh)));
+        const short2 a = __builtin_bit_cast(short2, w.x), b = __builtin_bit_cast(short2, w.y);
+        tile[jj][kk] = detect_one(make_short2((short)sat16(a.x >> sh), (short)sat16(a.y >> sh)));
+        tile[jj + 1][kk] = detect_one(make_short2((short)sat16(b.x >> sh), (short)sat16(b.y >> sh)));
     }
     __syncthreads();
-    for (int r = threadIdx.x; r < 64 * 64; r += 256) {
-        const int jj = r / 64, kk = r % 64;
-        out[(size_t)(j0 + jj) * G + k0 + kk] = tile[jj][kk];
+#pragma unroll
+    for (int r = threadIdx.x; r < 64 * 32; r += 256) {
+        const int jj = r / 32, kk = (r % 32) * 2;
+        const unsigned pr = (unsigned)tile[jj][kk] | ((unsigned)tile[jj][kk + 1] << 16);
+        *reinterpret_cast<unsigned *>(out + (size_t)(j0 + jj) * G + k0 + kk) = pr;
     }
 }
 
