@@ -127,7 +127,8 @@ __global__ __launch_bounds__(af_fb_nb<L>()) void k_af_fftb_adj(const float2 *__r
         if ((r + c) & 1) v = make_float2(-v.x, -v.y);
         return v;
     };
-    pfa32::fft_src<L, !INVF, AF_FB_EMAX>(buf, tw, src, [&](int j, float2 v) {
+    // the sink writes global memory and the kernel ends: no barriers in the last pass
+    pfa32::fft_src<L, !INVF, AF_FB_EMAX, false, false>(buf, tw, src, [&](int j, float2 v) {
         if (j < n) B[bblk(r, j, gridDim.x)] = make_float2(v.x * scale, v.y * scale);
     });
 }

@@ -140,8 +140,12 @@ __device__ __forceinline__ float2 call_src(Src &src, int e, int s) {
 }
 
 // One Stockham pass over the LDS line `buf`; LS = product of earlier radices.
-// On the last pass the outputs go to sink(index, value) instead of LDS.
-template <int L, int EMAX, int R, int LS, bool INV, bool LAST, typename Src, typename Sink>
+// On the last pass the outputs go to sink(index, value) instead of LDS.  That
+// pass writes nothing to LDS itself, so its reads need a barrier only when the
+// sink writes the line (SINK_LDS), and its closing barrier only when the
+// caller writes the line next without one of its own (END_SYNC).
+template <int L, int EMAX, int R, int LS, bool INV, bool LAST, typename Src, typename Sink,
+          bool SINK_LDS = true, bool END_SYNC = true>
 __device__ __forceinline__ void pass(float2 *buf, const Tw &tw, Src *src, Sink &sink) {
     using C = Cfg<L, EMAX>;
     constexpr int N = C::N, NT = C::NT, B = C::E / R;   // butterflies per thread
@@ -167,7 +171,7 @@ __device__ __forceinline__ void pass(float2 *buf, const Tw &tw, Src *src, Sink &
                 x[b][r] = src ? call_src(*src, k + r * NR, b + r * B) : rb[r * NR + (r * NR) / 32];
         }
     }
-    if (!src) __syncthreads();
+    if (!src && (!LAST || SINK_LDS)) __syncthreads();
     if (t < NT) {
 #pragma unroll
         for (int b = 0; b < B; ++b) {
@@ -217,16 +221,18 @@ __device__ __forceinline__ void pass(float2 *buf, const Tw &tw, Src *src, Sink &
             }
         }
     }
-    __syncthreads();
+    if (!LAST || END_SYNC) __syncthreads();
 }
 
-template <int L, int EMAX, bool INV, int K, int LS, typename Src, typename Sink>
+template <int L, int EMAX, bool INV, int K, int LS, bool SINK_LDS, bool END_SYNC, typename Src,
+          typename Sink>
 __device__ __forceinline__ void fft_from(float2 *buf, const Tw &tw, Src *src, Sink &sink) {
     using C = Cfg<L, EMAX>;
     if constexpr (K < C::passes()) {
         constexpr int R = C::radix(K);
-        pass<L, EMAX, R, LS, INV, K == C::passes() - 1>(buf, tw, K == 0 ? src : (Src *)nullptr, sink);
-        fft_from<L, EMAX, INV, K + 1, LS * R>(buf, tw, src, sink);
+        pass<L, EMAX, R, LS, INV, K == C::passes() - 1, Src, Sink, SINK_LDS, END_SYNC>(
+            buf, tw, K == 0 ? src : (Src *)nullptr, sink);
+        fft_from<L, EMAX, INV, K + 1, LS * R, SINK_LDS, END_SYNC>(buf, tw, src, sink);
     }
 }
 
@@ -238,14 +244,32 @@ struct NoSrc {
 // delivers (index, value) to `sink` (which may write back to buf).
 template <int L, bool INV, int EMAX = 32, typename Sink>
 __device__ __forceinline__ void fft(float2 *buf, const Tw &tw, Sink &&sink) {
-    fft_from<L, EMAX, INV, 0, 1>(buf, tw, (NoSrc *)nullptr, sink);
+    fft_from<L, EMAX, INV, 0, 1, true, true>(buf, tw, (NoSrc *)nullptr, sink);
+}
+
+// The passes after a radix-G first pass that the caller ran in registers and
+// stored to the line (outputs G k + r of the butterfly over k + r N/G): radix
+// min(E, N / LS) from LS = G on.  The last pass runs without barriers
+// (the sink writes global memory, and the caller synchronises before it
+// writes the line again).
+template <int L, int EMAX, bool INV, int LS, typename Sink>
+__device__ __forceinline__ void fft_rest(float2 *buf, const Tw &tw, Sink &sink) {
+    constexpr int N = 1 << L, E = Cfg<L, EMAX>::E;
+    if constexpr (LS < N) {
+        constexpr int R = N / LS < E ? N / LS : E;
+        pass<L, EMAX, R, LS, INV, LS * R == N, NoSrc, Sink, false, false>(buf, tw, (NoSrc *)nullptr, sink);
+        fft_rest<L, EMAX, INV, LS * R>(buf, tw, sink);
+    }
 }
 
 // As fft, but the first pass loads element e with src(e) (e.g. from global
-// memory) instead of from the LDS line.
-template <int L, bool INV, int EMAX = 32, typename Src, typename Sink>
+// memory) instead of from the LDS line.  SINK_LDS / END_SYNC as for pass: a
+// sink that writes only global memory and a caller that synchronises before
+// it next writes the line may drop the last pass's barriers.
+template <int L, bool INV, int EMAX = 32, bool SINK_LDS = true, bool END_SYNC = true, typename Src,
+          typename Sink>
 __device__ __forceinline__ void fft_src(float2 *buf, const Tw &tw, Src &&src, Sink &&sink) {
-    fft_from<L, EMAX, INV, 0, 1>(buf, tw, &src, sink);
+    fft_from<L, EMAX, INV, 0, 1, SINK_LDS, END_SYNC>(buf, tw, &src, sink);
 }
 
 }  // namespace pfa32

@@ -544,6 +544,8 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
     constexpr int G = (M2 + 4 * NB - 1) / (4 * NB);   // 4-output groups per thread
     constexpr int PR = (M2 + NB - 1) / NB;             // row samples per thread
     constexpr bool RES = MODE != 2;                    // resampling modes
+    // the fused FP32 form runs the line FFT's first pass (radix G) in registers
+    constexpr bool R1 = MODE == 1 && FK == FK_F32 && G >= 2 && 4 * G * NB == M2;
     __shared__ float2 reg[(M2 + M2 / 16 + 64 > Cf::PADN + 64) ? M2 + M2 / 16 + 64 : Cf::PADN + 64];  // positions | row | FFT line
     __shared__ unsigned tp[(SINC && RES && !BIG) ? TPWORDS : 1];
     __shared__ float2 twl[M2 / 64 + 64];
@@ -820,15 +822,34 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
                 continue;
             }
             __syncthreads();                          // the row is dead: the FFT line takes over
-#pragma unroll
-            for (int g = 0; g < G; ++g) {
+            if constexpr (R1) {
+                // the FFT's first pass in registers: the thread's groups hold
+                // points k + g N/G (k = 4 tl + u), the inputs of one radix-G
+                // butterfly (LS = 1, no twiddles); outputs G k + r go to the line
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const int q = (g * NB + tl) * 4 + u;
-                    if (q < M2) {
-                        float2 w = v[g * 4 + u];
-                        if ((q + j) & 1) w = make_float2(-w.x, -w.y);
-                        reg[pfa32::padf(q)] = w;
+                    const int k = 4 * tl + u;
+                    float2 x[G];
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {     // (-1)^(q + j): N/G is even
+                        x[g] = v[g * 4 + u];
+                        if ((k + j) & 1) x[g] = make_float2(-x[g].x, -x[g].y);
+                    }
+                    pfa32::dft<G, INV>(x);
+#pragma unroll
+                    for (int r = 0; r < G; ++r) reg[pfa32::padf(G * k + r)] = x[r];
+                }
+            } else {
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int q = (g * NB + tl) * 4 + u;
+                        if (q < M2) {
+                            float2 w = v[g * 4 + u];
+                            if ((q + j) & 1) w = make_float2(-w.x, -w.y);
+                            reg[pfa32::padf(q)] = w;
+                        }
                     }
                 }
             }
@@ -837,10 +858,14 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
             __syncthreads();
             const int nj = a_rows(n);
             const float scale = INV ? 1.0f / (float)M2 : 1.0f;
-            pfa32::fft<L, INV, AZ_FFT_EMAX>(reg, tw, [&](int idx, float2 w) {
+            auto sink = [&](int idx, float2 w) {
                 if (INV) w = make_float2(w.x * scale, w.y * scale);
                 st_stream(out + ablk(j, idx, nj), w);
-            });
+            };
+            // the last pass runs without barriers: the next line synchronises
+            // before it writes the LDS region
+            if constexpr (R1) pfa32::fft_rest<L, AZ_FFT_EMAX, INV, G>(reg, tw, sink);
+            else pfa32::fft<L, INV, AZ_FFT_EMAX>(reg, tw, sink);
         }
     }
     count_wave(nslow ? nslow + 1 : nullptr, slow);
@@ -893,8 +918,10 @@ __global__ __launch_bounds__(fb2_nb<L>(), 4) void k_fft_b2_f32(
             }
         };
     };
-    pfa32::fft_src<L, INV, FB_EMAX>(buf, tw, [&](int, int s2) { return c0[s2]; }, sink_for(r0));
-    pfa32::fft_src<L, INV, FB_EMAX>(buf, tw, [&](int, int s2) { return c1[s2]; }, sink_for(r0 + 1));
+    // the sinks write global memory: the last passes skip their read barrier,
+    // and the second transform its closing one (the kernel ends)
+    pfa32::fft_src<L, INV, FB_EMAX, false, true>(buf, tw, [&](int, int s2) { return c0[s2]; }, sink_for(r0));
+    pfa32::fft_src<L, INV, FB_EMAX, false, false>(buf, tw, [&](int, int s2) { return c1[s2]; }, sink_for(r0 + 1));
 }
 
 // Two-level twiddle tables of an n-point line: hi[k] = W^(64 k), k < n/64,
