@@ -137,8 +137,13 @@ __global__ __launch_bounds__(af_fb_nb<L>()) void k_af_fftb_adj(const float2 *__r
 // gradient partials, persistent over range bins as the forward azimuth kernel.
 // Thread t owns sources p = 4 (g NB + t) + u and accumulates their gradient
 // over its lines in registers; partial[blockIdx][p] at the end.
+// M2 / 4 threads (1024 at 4096 points): one group of four sources and four
+// outputs a thread, 16 waves a CU to cover the LDS latencies of the
+// segmented sums; the line FFT runs on its own thread count within them.
+template <int L>
+constexpr int af_adj_nb() { return (1 << L) / 4 < 64 ? 64 : ((1 << L) / 4 > 1024 ? 1024 : (1 << L) / 4); }
 template <int L, bool SINC, bool INVF>
-__global__ __launch_bounds__(az_nb<L>()) void k_af_az_adj(
+__global__ __launch_bounds__(af_adj_nb<L>()) void k_af_az_adj(
     const float2 *__restrict__ g1t, int ldp, int m, int n, const Grid1 KR,
     const double *__restrict__ tan_s, const TanBins tb, const Grid1 KC,
     const double *__restrict__ hm, const double *__restrict__ hn,
@@ -147,14 +152,19 @@ __global__ __launch_bounds__(az_nb<L>()) void k_af_az_adj(
     constexpr int EM = az_emax<L>();
     using Cf = pfa32::Cfg<L, EM>;
     constexpr int M2 = Cf::N;
-    constexpr int NB = Cf::NB;
+    constexpr int NB = af_adj_nb<L>();
     constexpr int G = (M2 + 4 * NB - 1) / (4 * NB);
     constexpr int PQ = (M2 + NB - 1) / NB;
+    // first[p + FO] = the first output q whose bracket count c(q) >= p, for
+    // p in [-FO, M2 + FO): the outputs whose windows cover sources p0 .. p0 + 3
+    // are q in [first[p0 - 15], first[p0 + 20]) (c non-decreasing in q)
+    constexpr int FO = 32;
     __shared__ float2 reg[Cf::PADN + 64];              // positions | adjoint line
     __shared__ float tapl[SINC ? TROWS * 256 : 1];
     __shared__ float2 twl[M2 / 64 + 64];
     __shared__ int code[M2];                           // (c << 8) | phase of every output
     __shared__ float frq[M2];
+    __shared__ int first[M2 + 2 * FO];
     double *xs = reinterpret_cast<double *>(reg);
     const int t = threadIdx.x;
     if (SINC) load_taps(tapl, sinc);
@@ -163,9 +173,20 @@ __global__ __launch_bounds__(az_nb<L>()) void k_af_az_adj(
     const int ranks = (gridDim.x + 7) / 8;
     const int x = blockIdx.x % 8, rank = blockIdx.x / 8;
     const float scale = INVF ? 1.0f / (float)M2 : 1.0f;
+    constexpr int NTF = Cf::NT, EF = Cf::E;            // the line FFT's threads and elements a thread
     float acc[G * 4];
 #pragma unroll
     for (int k = 0; k < G * 4; ++k) acc[k] = 0.f;
+    // line-independent values held in registers: the tan values of this
+    // thread's source positions and the phase rotation of its sources.
+    // Loads are unconditional, from clamped indexes (a load on a path taken
+    // at run time is waited for where the paths join).
+    double tsv[PQ];
+#pragma unroll
+    for (int k = 0; k < PQ; ++k) tsv[k] = tan_s[min(t + k * NB, m - 1)];
+    float2 rv[G * 4];
+#pragma unroll
+    for (int k = 0; k < G * 4; ++k) rv[k] = rot[min(4 * ((k / 4) * NB + t) + k % 4, m - 1)];
     for (int base = 0; base < per_x; base += ranks) {
         const int jl = base + rank;
         const int j = x * per_x + jl;
@@ -174,8 +195,21 @@ __global__ __launch_bounds__(az_nb<L>()) void k_af_az_adj(
         asm volatile("" : "+v"(tl));
         const double krj = KR(j);
         const double inv_krj = 1.0 / krj;
+        // this line's global reads, issued before its first barrier: the
+        // bracket guesses, the FFT's column of B and the g1t row
+        int gix[PQ];
+#pragma unroll
+        for (int k = 0; k < PQ; ++k) gix[k] = tb.guess(__dmul_rn(KC(min(tl + k * NB, m - 1)), inv_krj));
+        float2 bv[EF];
+#pragma unroll
+        for (int s2 = 0; s2 < EF; ++s2) bv[s2] = B[bblk(min(tl, NTF - 1) + s2 * NTF, j, M2)];
+        float2 gv[G * 4];
+#pragma unroll
+        for (int k = 0; k < G * 4; ++k) gv[k] = g1t[(size_t)j * ldp + min(4 * ((k / 4) * NB + tl) + k % 4, m - 1)];
         __syncthreads();                               // previous line fully consumed
-        for (int p = tl; p < m; p += NB) xs[p] = __dmul_rn(krj, tan_s[p]);
+#pragma unroll
+        for (int k = 0; k < PQ; ++k)
+            if (tl + k * NB < m) xs[tl + k * NB] = __dmul_rn(krj, tsv[k]);
         __syncthreads();
         // brackets of every output, as the forward computes them
 #pragma unroll
@@ -188,7 +222,7 @@ __global__ __launch_bounds__(az_nb<L>()) void k_af_az_adj(
                 continue;
             }
             const double Q = KC(q);
-            int c = tb.guess(__dmul_rn(Q, inv_krj));
+            int c = gix[k];
             double xl = c > 0 ? xs[c - 1] : -INFINITY;
             double xh = c < m ? xs[c] : INFINITY;
             while (c > 0 && !(xl <= Q)) { --c; xh = xl; xl = c > 0 ? xs[c - 1] : -INFINITY; }
@@ -206,8 +240,20 @@ __global__ __launch_bounds__(az_nb<L>()) void k_af_az_adj(
             frq[q] = fr;
         }
         __syncthreads();                               // positions dead: the FFT line takes over
+        // first[]: output q fills the counts in (c(q - 1), c(q)]; the last
+        // output also fills the counts past its own with M2
+#pragma unroll
+        for (int k = 0; k < PQ; ++k) {
+            const int q = tl + k * NB;
+            if (q >= M2) continue;
+            const int c = min(code[q] >> 8, M2 + FO - 1);
+            const int cp = q > 0 ? min(code[q - 1] >> 8, M2 + FO - 1) : -FO - 1;
+            for (int pp = cp + 1; pp <= c; ++pp) first[pp + FO] = q;
+            if (q == M2 - 1)
+                for (int pp = c + 1; pp < M2 + FO; ++pp) first[pp + FO] = M2;
+        }
         const double hj = hn ? hn[j] : 1.0;
-        pfa32::fft_src<L, !INVF, EM>(reg, tw, [&](int r) { return B[bblk(r, j, M2)]; },
+        pfa32::fft_src<L, !INVF, EM>(reg, tw, [&](int, int s2) { return bv[s2]; },
             [&](int q, float2 v) {
                 if (q < m) {
                     float w = scale;
@@ -225,18 +271,7 @@ __global__ __launch_bounds__(az_nb<L>()) void k_af_az_adj(
             const int p0 = 4 * (g * NB + tl);
             if (p0 >= m) continue;
             // outputs with bracket count c in [p0 - 15, p0 + 19] (J in [p0 - 16, p0 + 18])
-            int lo = 0, hi = M2;
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if ((code[mid] >> 8) < p0 - 15) lo = mid + 1; else hi = mid;
-            }
-            const int qa = lo;
-            hi = M2;
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if ((code[mid] >> 8) <= p0 + 19) lo = mid + 1; else hi = mid;
-            }
-            const int qb = lo;
+            const int qa = first[p0 - 15 + FO], qb = first[p0 + 20 + FO];
             float2 H[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) H[u] = make_float2(0.f, 0.f);
@@ -291,7 +326,7 @@ __global__ __launch_bounds__(az_nb<L>()) void k_af_az_adj(
             for (int u = 0; u < 4; ++u) {
                 const int p = p0 + u;
                 if (p < m) {
-                    const float2 g1 = pfa32::cmul(g1t[(size_t)j * ldp + p], rot[p]);
+                    const float2 g1 = pfa32::cmul(gv[g * 4 + u], rv[g * 4 + u]);
                     acc[g * 4 + u] += H[u].y * g1.x - H[u].x * g1.y;
                 }
             }
@@ -397,7 +432,7 @@ int launch_af_az_adj(pfa_plan *p, const void *g1t, int ldp, const void *rot, con
 #define ARGS (const float2 *)g1t, ldp, p->m, p->n, grid_of(p->KR, p->kr_lin, p->n), p->tan_s64, tb, \
              grid_of(p->KC, p->kc_lin, p->m), p->hm, p->hn, p->sinc, (const float2 *)rot,           \
              (const float2 *)B, tw, partial
-#define K(LL, S, I) hipLaunchKernelGGL((k_af_az_adj<LL, S, I>), dim3(grid), dim3(az_nb<LL>()), 0, ctx->stream, ARGS)
+#define K(LL, S, I) hipLaunchKernelGGL((k_af_az_adj<LL, S, I>), dim3(grid), dim3(af_adj_nb<LL>()), 0, ctx->stream, ARGS)
 #define LAUNCH(LL)                          \
     case LL:                                \
         if (sinc && inv) K(LL, true, true);  \

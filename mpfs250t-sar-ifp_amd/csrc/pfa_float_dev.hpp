@@ -210,10 +210,11 @@ __device__ __forceinline__ int bracket_from(XA &xa, int S, double Q, int g, doub
 // along range so strip halos are L2 hits).  Per tile: every pulse's source
 // strip goes to LDS while each thread finds the FP64 brackets of its 8 bins
 // (closed-form guess from the uniform frequency grid, exact comparisons);
-// then four-bin shared-window sinc sums (or np.interp) and a transpose
-// through LDS so the corner-turned g1t rows are written 128 bytes at a time.
+// then four-bin shared-window sinc sums (or np.interp) stored straight to
+// the corner-turned g1t rows.
 // Lane layout: thread t serves pulse t % 16 and bin groups t / 16 and
-// t / 16 + 32, so the lanes of a wave read 16 different strips.
+// t / 16 + 32, so the lanes of a wave read 16 different strips and each
+// store writes 128 contiguous bytes of four g1t rows.
 template <bool SINC, bool FULLFREQ>
 __global__ __launch_bounds__(RT, 2) void k_range_f32(
     const float2 *__restrict__ sig, int m, int n, const int32_t *__restrict__ order,
@@ -221,16 +222,14 @@ __global__ __launch_bounds__(RT, 2) void k_range_f32(
     const double *__restrict__ dfv, const double *__restrict__ freq, const Grid1 KR,
     const int16_t *__restrict__ sinc, float2 *__restrict__ g1t, int ldp,
     unsigned *__restrict__ nslow) {
-    static_assert(sizeof(float2) * RQ * (RP + 1) <= sizeof(float2) * RP * RSP, "tile fits the strips");
     static_assert(RQ == 4 * 2 * (RT / RP), "two groups of four bins per thread");
-    __shared__ float2 strip[RP * RSP];                 // source strips, then the output tile
+    __shared__ float2 strip[RP * RSP];                 // source strips
     __shared__ unsigned tp[SINC ? TPWORDS : 1];
     __shared__ int lo_s[RP], hi_s[RP], st_s[RP], dir_s[RP];
     __shared__ int ps_i[RP];                          // per pulse of the current group: sorted index
     __shared__ double ps_a[RP], ps_f0[RP], ps_df[RP], ps_pr[RP];  // 2 pr / c, f0, df, pr
     __shared__ double ps_x0[RP], ps_iad[RP];          // a f0, 1 / (a df) (bracket guesses)
     __shared__ int ps_asc[RP];                        // ascending positions
-    auto tile = reinterpret_cast<float2 (*)[RP + 1]>(&strip[0]);
     if (SINC) load_taps2(tp, sinc);
     const int tiles_p = (m + RP - 1) / RP, tiles_q = (n + RQ - 1) / RQ;
     const int my_groups = blockIdx.x < tiles_p ? (tiles_p - 1 - blockIdx.x) / gridDim.x + 1 : 0;
@@ -435,23 +434,15 @@ __global__ __launch_bounds__(RT, 2) void k_range_f32(
                 }
             }
         }
-        __syncthreads();                              // strips dead: the tile takes over
+        // lanes 16 i .. 16 i + 15 hold 16 consecutive pulses of one range bin:
+        // each store writes 128 contiguous bytes of a g1t row
 #pragma unroll
         for (int g = 0; g < 2; ++g)
 #pragma unroll
-            for (int u = 0; u < 4; ++u) tile[4 * (gi + g * (RT / RP)) + u][pl] = v[g * 4 + u];
-        __syncthreads();
-        {                                             // all tile reads first, then the stores
-            constexpr int NS = RQ * RP / RT, QS = RT / RP;
-            const int qq = t / RP, pp = t % RP;
-            float2 w[NS];
-#pragma unroll
-            for (int i = 0; i < NS; ++i) w[i] = tile[qq + i * QS][pp];
-            float2 *dst = g1t + (size_t)(q0 + qq) * ldp + p0 + pp;
-#pragma unroll
-            for (int i = 0; i < NS; ++i)
-                if (q0 + qq + i * QS < n && p0 + pp < m) st_stream(dst + (size_t)(i * QS) * ldp, w[i]);
-        }
+            for (int u = 0; u < 4; ++u) {
+                const int q = q0 + 4 * (gi + g * (RT / RP)) + u;
+                if (q < qe && p < m) st_stream(g1t + (size_t)q * ldp + p, v[g * 4 + u]);
+            }
     }
     count_wave(nslow, slow);
 }
