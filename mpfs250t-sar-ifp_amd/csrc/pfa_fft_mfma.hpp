@@ -86,13 +86,16 @@ __device__ __forceinline__ float2 unpack(unsigned w) {
 }
 
 // Radix plan: stage radices (first stage first).
+// LS0: the product of the radices the caller has already applied in
+// registers before it puts the line (1: none; 2: a radix-2 first pass,
+// see put_r2).
 template <int L>
 struct MPlan;
-template <> struct MPlan<8>  { static constexpr int NS = 2; static constexpr int R[3] = {16, 16, 1}; };
-template <> struct MPlan<9>  { static constexpr int NS = 2; static constexpr int R[3] = {32, 16, 1}; };
-template <> struct MPlan<10> { static constexpr int NS = 2; static constexpr int R[3] = {32, 32, 1}; };
-template <> struct MPlan<12> { static constexpr int NS = 3; static constexpr int R[3] = {16, 16, 16}; };
-template <> struct MPlan<13> { static constexpr int NS = 3; static constexpr int R[3] = {32, 16, 16}; };
+template <> struct MPlan<8>  { static constexpr int NS = 2, LS0 = 1; static constexpr int R[3] = {16, 16, 1}; };
+template <> struct MPlan<9>  { static constexpr int NS = 2, LS0 = 1; static constexpr int R[3] = {32, 16, 1}; };
+template <> struct MPlan<10> { static constexpr int NS = 2, LS0 = 1; static constexpr int R[3] = {32, 32, 1}; };
+template <> struct MPlan<12> { static constexpr int NS = 3, LS0 = 1; static constexpr int R[3] = {16, 16, 16}; };
+template <> struct MPlan<13> { static constexpr int NS = 3, LS0 = 2; static constexpr int R[3] = {16, 16, 16}; };
 
 template <int L>
 constexpr bool supported() { return L == 8 || L == 9 || L == 10 || L == 12 || L == 13; }
@@ -111,7 +114,7 @@ constexpr int nw_for() {
 
 template <int L, int S>
 constexpr int ls_of() {                      // product of the radices before stage S
-    int p = 1;
+    int p = MPlan<L>::LS0;
     for (int s = 0; s < S; ++s) p *= MPlan<L>::R[s];
     return p;
 }
@@ -267,7 +270,7 @@ __device__ __forceinline__ void stage(unsigned *planes, const pfa32::Tw &tw, con
             acc[g][mt] = c;
         }
     }
-    __syncthreads();                                   // every operand read before any write
+    if (!LAST) __syncthreads();                        // every operand read before any write
 #pragma unroll
     for (int g = 0; g < GPW; ++g) {
         if (!active) break;
@@ -325,7 +328,7 @@ __device__ __forceinline__ void stage(unsigned *planes, const pfa32::Tw &tw, con
             }
         }
     }
-    __syncthreads();
+    if (!LAST) __syncthreads();
 }
 
 template <int L, bool BF, bool INV, bool SPLIT, int NCOL, int NW, int S, typename Ops, typename Sink>
@@ -351,9 +354,29 @@ template <int L, bool BF>
 __device__ __forceinline__ void put_at(unsigned *base, unsigned d, float2 v) {
     base[pp((int)pos_of<L, 0>(d))] = pack<BF>(v);
 }
+// Plans with LS0 = 2: the radix-2 first pass of points a = x[k], b = x[k + N/2]
+// (k < N/2), run by the caller in FP32 and put as the first stage's input:
+// outputs (a + b) / 2 and (a - b) / 2 (the 1/2 keeps the transform at DFT/N)
+// at o = 2k and 2k + 1, the odd one times the first stage's twiddle
+// W_N^(r N / 32) = W_32^r, r = o / (N / 16) (j = o % 2 = 1 of LS = 2, step N / 32).
+template <int L, bool BF, bool INV>
+__device__ __forceinline__ void put_r2(unsigned *line, unsigned k, float2 a, float2 b) {
+    static_assert(MPlan<L>::LS0 == 2 && MPlan<L>::R[0] == 16, "a radix-2 first pass ahead of radix 16");
+    constexpr unsigned NV = (1u << L) / 16;
+    const float2 y0 = make_float2((a.x + b.x) * 0.5f, (a.y + b.y) * 0.5f);
+    float2 y1 = make_float2((a.x - b.x) * 0.5f, (a.y - b.y) * 0.5f);
+    const unsigned o1 = 2 * k + 1;
+    const int r = (int)(o1 / NV);
+    const float c = pfa32::kC32[r][0], sn = INV ? pfa32::kC32[r][1] : -pfa32::kC32[r][1];
+    y1 = make_float2(fmaf(y1.x, c, -y1.y * sn), fmaf(y1.x, sn, y1.y * c));
+    line[pp((int)pos_of<L, 0>(2 * k))] = pack<BF>(y0);
+    line[pp((int)pos_of<L, 0>(o1))] = pack<BF>(y1);
+}
 
 // Transform NCOL lines whose first-stage layout is in `planes` (after a
 // barrier); sink(col, index, value) receives DFT/N (or IDFT/N) in FP32.
+// The last stage has no barriers: the sink must not write `planes`, and the
+// caller synchronises before it writes `planes` again.
 template <int L, bool BF, bool INV, bool SPLIT, int NCOL, int NW, typename Ops, typename Sink>
 __device__ __forceinline__ void fft(unsigned *planes, const pfa32::Tw &tw, const Ops &ops,
                                     Sink &&sink) {

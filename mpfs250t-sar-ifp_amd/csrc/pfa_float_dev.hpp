@@ -791,15 +791,30 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
                 const int se = pfam::norm_exp(lm);
                 const float sc = ldexpf(1.0f, -se);
                 unsigned *planes = reinterpret_cast<unsigned *>(reg);
-                unsigned *pb = pfam::put_base<L>(planes, 0, 4 * tl);
                 static_assert(4 * G * NB == M2, "the threads' groups cover the line");
+                if constexpr (pfam::MPlan<L>::LS0 == 2) {
+                    // the radix-2 first pass in registers: groups g and g + G/2
+                    // hold points k and k + N/2 (k = 4 (g NB + tl) + u)
 #pragma unroll
-                for (int g = 0; g < G; ++g) {
+                    for (int g = 0; g < G / 2; ++g) {
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {          // q = (g NB + tl) 4 + u
-                        float2 w = make_float2(v[g * 4 + u].x * sc, v[g * 4 + u].y * sc);
-                        if ((u + j) & 1) w = make_float2(-w.x, -w.y);   // (-1)^(q + j)
-                        pfam::put_at<L, BF>(pb, 4 * g * NB + u, w);
+                        for (int u = 0; u < 4; ++u) {
+                            const float s = ((u + j) & 1) ? -sc : sc;        // (-1)^(q + j) 2^-se
+                            const float2 a = make_float2(v[g * 4 + u].x * s, v[g * 4 + u].y * s);
+                            const float2 b = make_float2(v[(g + G / 2) * 4 + u].x * s, v[(g + G / 2) * 4 + u].y * s);
+                            pfam::put_r2<L, BF, INV>(planes, 4 * (g * NB + tl) + u, a, b);
+                        }
+                    }
+                } else {
+                    unsigned *pb = pfam::put_base<L>(planes, 0, 4 * tl);
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {          // q = (g NB + tl) 4 + u
+                            float2 w = make_float2(v[g * 4 + u].x * sc, v[g * 4 + u].y * sc);
+                            if ((u + j) & 1) w = make_float2(-w.x, -w.y);   // (-1)^(q + j)
+                            pfam::put_at<L, BF>(pb, 4 * g * NB + u, w);
+                        }
                     }
                 }
                 __syncthreads();

@@ -79,18 +79,25 @@ __global__ __launch_bounds__(fftb_mfma_nb<L>()) void k_fft_b_mfma(
 #pragma unroll
     for (int i = 0; i < BW; ++i) S = max(S, emx[i]);
     S = (S == INT_MIN) ? 0 : S + 2;
-    unsigned *pb = pfam::put_base<L>(planes, 0, t);
-#pragma unroll
-    for (int i = 0; i < NI; ++i) {
-        const int jj = row_of(i);
-        const bool in = jj < n;
+    auto value = [&](int i, int c) {
         const float sc = ldexpf(1.0f, ex[i] - S);
+        const float2 v = pfam::unpack<BF>(a[i][c]);
+        return row_of(i) < n ? make_float2(v.x * sc, v.y * sc) : make_float2(0.f, 0.f);
+    };
+    if constexpr (pfam::MPlan<L>::LS0 == 2) {
+        // the radix-2 first pass in registers: rows i and i + NI/2 are points
+        // k and k + N/2 (k = t + i NB)
 #pragma unroll
-        for (int c = 0; c < NCOL; ++c) {
-            const float2 v = pfam::unpack<BF>(a[i][c]);
-            pfam::put_at<L, BF>(pb + c * PL, i * NB,
-                                in ? make_float2(v.x * sc, v.y * sc) : make_float2(0.f, 0.f));
-        }
+        for (int i = 0; i < NI / 2; ++i)
+#pragma unroll
+            for (int c = 0; c < NCOL; ++c)
+                pfam::put_r2<L, BF, INV>(planes + c * PL, row_of(i), value(i, c), value(i + NI / 2, c));
+    } else {
+        unsigned *pb = pfam::put_base<L>(planes, 0, t);
+#pragma unroll
+        for (int i = 0; i < NI; ++i)
+#pragma unroll
+            for (int c = 0; c < NCOL; ++c) pfam::put_at<L, BF>(pb + c * PL, i * NB, value(i, c));
     }
     __syncthreads();
     const float osc = ldexpf(1.0f, S + (INV ? 0 : L));
