@@ -568,9 +568,9 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
     };
     if (RES && active_at(0)) prefetch(line_of(0));
     // the tan values of this thread's source positions are the same for every
-    // line: held in registers where they fit (the FP32 form of lines up to
-    // 8192 points), loaded per line otherwise
-    constexpr bool HOLD = RES && !BIG && FK == FK_F32;
+    // line: held in registers where they fit (lines up to 8192 points, every
+    // FFT form), loaded per line otherwise
+    constexpr bool HOLD = RES && !BIG;
     double tsv[HOLD ? PR : 1];
     if constexpr (HOLD) {
 #pragma unroll
