@@ -240,17 +240,24 @@ __global__ __launch_bounds__(af_adj_nb<L>()) void k_af_az_adj(
             frq[q] = fr;
         }
         __syncthreads();                               // positions dead: the FFT line takes over
-        // first[]: output q fills the counts in (c(q - 1), c(q)]; the last
-        // output also fills the counts past its own with M2
+        // first[]: output q >= 1 fills the counts in (c(q - 1), c(q)] (a few
+        // each); the counts up to c(0) (output 0) and past c(M2 - 1) (none:
+        // M2), which can be long runs, are filled by all threads together
+        {
+            const int c0 = min(code[0] >> 8, M2 + FO - 1), cl = min(code[M2 - 1] >> 8, M2 + FO - 1);
+            for (int i = tl; i < M2 + 2 * FO; i += NB) {
+                const int pp = i - FO;
+                if (pp <= c0) first[i] = 0;
+                else if (pp > cl) first[i] = M2;
+            }
+        }
 #pragma unroll
         for (int k = 0; k < PQ; ++k) {
             const int q = tl + k * NB;
-            if (q >= M2) continue;
+            if (q < 1 || q >= M2) continue;
             const int c = min(code[q] >> 8, M2 + FO - 1);
-            const int cp = q > 0 ? min(code[q - 1] >> 8, M2 + FO - 1) : -FO - 1;
+            const int cp = min(code[q - 1] >> 8, M2 + FO - 1);
             for (int pp = cp + 1; pp <= c; ++pp) first[pp + FO] = q;
-            if (q == M2 - 1)
-                for (int pp = c + 1; pp < M2 + FO; ++pp) first[pp + FO] = M2;
         }
         const double hj = hn ? hn[j] : 1.0;
         pfa32::fft_src<L, !INVF, EM>(reg, tw, [&](int, int s2) { return bv[s2]; },
