@@ -173,6 +173,15 @@ struct Fft {
     }
 };
 
+// The pass-A output ya [range row j][cross bin k] (G x G) in blocks of 16
+// cross bins: element (j, k) at ((k / 16) G + j) 16 + k % 16.  FFT-B reads a
+// column down consecutive rows 64 bytes apart (16 columns share each chunk, and
+// their workgroups run on one XCD), not a full row (4 G bytes) apart; pass A
+// still writes whole 64-byte chunks.
+__device__ __forceinline__ size_t yablk(int j, int k, int G) {
+    return ((size_t)(k >> 4) * G + j) * 16 + (k & 15);
+}
+
 template <int L>
 __device__ __forceinline__ void load_twiddles(short2 *tw, const short2 *twg) {
     constexpr int H = (1 << L) / 2;
@@ -745,8 +754,7 @@ __global__ __launch_bounds__(Fft<L>::NB) void k_azimuth_fft(
         if (t < F::NT) {
 #pragma unroll
             for (int l = 0; l < F::E; ++l)
-                out[(size_t)j * G + F::idx(t, l, L - F::P)] =
-                    make_short2((short)xr[l], (short)xi[l]);
+                out[yablk(j, F::idx(t, l, L - F::P), G)] = make_short2((short)xr[l], (short)xi[l]);
         }
         if (t == 0) {
             e_row[j] = e;
@@ -776,10 +784,21 @@ __global__ __launch_bounds__(Fft<L>::NB) void k_fft_b(const short2 *__restrict__
     const int per_xcd = G / 8;
     const int k = (G >= 8) ? (b % 8) * per_xcd + b / 8 : b;
     const int ema = *emax_a;
-    for (int r = threadIdx.x; r < G; r += blockDim.x) {
-        const short2 v = ya[(size_t)r * G + k];
-        const int sh = ema - ea[r];
-        data[pad32(r)] = make_short2((short)sat16(v.x >> sh), (short)sat16(v.y >> sh));
+    // every row load of the column in flight before the first is used
+    constexpr int PERB = G / F::NB;
+    static_assert(PERB * F::NB == G, "the threads' rows cover the column");
+    const int t = threadIdx.x;
+    short2 v[PERB];
+    int er[PERB];
+#pragma unroll
+    for (int i = 0; i < PERB; ++i) {
+        v[i] = ya[yablk(t + i * F::NB, k, G)];
+        er[i] = ea[t + i * F::NB];
+    }
+#pragma unroll
+    for (int i = 0; i < PERB; ++i) {
+        const int sh = ema - er[i];
+        data[pad32(t + i * F::NB)] = make_short2((short)sat16(v[i].x >> sh), (short)sat16(v[i].y >> sh));
     }
     load_twiddles<L>(tw, twg);
     __syncthreads();
