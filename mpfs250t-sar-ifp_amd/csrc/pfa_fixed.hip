@@ -7,7 +7,7 @@
 // Kernels
 //   k_range_resample    pass 1 (F3+F4+F5): FP64 positions, bracket, 2-tap lerp or
 //                       32-tap sinc, pulse-sorted rows written corner-turned
-//                       through an LDS tile: g1t[range q][sorted pulse p].
+//                       from registers: g1t[range q][sorted pulse p].
 //   k_azimuth_fft       pass 2 (F6+F7) + window (F8) + BFP FFT along cross (F9a),
 //                       one range row per workgroup, everything LDS-resident.
 //   k_fft_b             BFP FFT along range (F9b) for one cross column per
@@ -442,19 +442,17 @@ constexpr int SWP = SW + 1;  // odd pitch: the pulses of a wave start in differe
 //      staged in LDS by coalesced loads (the four threads of a pulse, lanes
 //      16 apart, agree on its start with two lane swaps);
 //  (3) the gathers from the strip (from the row in global memory for the
-//      rare window that leaves it), then the tile transposed through LDS so
-//      g1t rows get whole 256-byte runs.
+//      rare window that leaves it), stored straight to g1t: 16 pulses of one
+//      output are one 64-byte run of a row.
 template <bool SPLIT>
 __global__ __launch_bounds__(256) void k_range_resample(
     const short2 *__restrict__ sig, int m, int n, int grid, const float *__restrict__ a32,
     const float *__restrict__ f0v, const float *__restrict__ dfv,
     const int32_t *__restrict__ order, const float *__restrict__ krp,
     const int16_t *__restrict__ sinc, int flags, short2 *__restrict__ g1t, int ldp) {
-    static_assert(QT * (RT + 1) <= RT * SWP, "the output tile fits over the strips");
-    __shared__ short2 strip[RT * SWP];             // source strips, then the output tile
+    __shared__ short2 strip[RT * SWP];             // source strips
     __shared__ uint4 stl[256 * 4];                 // the 256 x 32 Q15 tap table, 16 KB
     __shared__ int lo_s[RT], row_s[RT];
-    auto tile = reinterpret_cast<short2 (*)[RT + 1]>(&strip[0]);
     const int p0 = blockIdx.x * RT, q0 = blockIdx.y * QT;
     const int t = threadIdx.x;
     const int pl = ((t >> 6) << 4) | (t & 15), ch = (t >> 4) & 3;
@@ -587,14 +585,12 @@ __global__ __launch_bounds__(256) void k_range_resample(
             out[u] = v;
         }
     }
-    __syncthreads();                               // strips dead: the tile takes over
+    // lanes 16 i .. 16 i + 15 hold 16 consecutive pulses of one range output:
+    // each store writes four 64-byte runs of g1t rows (the LDS transpose of the
+    // output tile this replaced cost two barriers and 2.5 % of the pass)
 #pragma unroll
-    for (int u = 0; u < PER; ++u) tile[ch * PER + u][pl] = out[u];
-    __syncthreads();
-    for (int r = t; r < QT * RT; r += 256) {
-        const int qq = r / RT, pp = r % RT;
-        if (p0 + pp < m) g1t[(size_t)(q0 + qq) * ldp + p0 + pp] = tile[qq][pp];
-    }
+    for (int u = 0; u < PER; ++u)
+        if (p0 + pl < m) g1t[(size_t)(q0 + ch * PER + u) * ldp + p0 + pl] = out[u];
 }
 
 // ------------------------------------------------------------------ pass 2 + window + FFT A
