@@ -11,7 +11,10 @@ through the decomposition the kernel uses, and requires them to agree.
       radix min(E, remaining), E = N / NT elements per thread.
   Matrix-core FFT (csrc/pfa_fft_mfma.hpp): pp(p) = p + 4 (p / 32) + 4 (p / 512),
       radix plans {256: 16.16, 512: 32.16, 1024: 32.32, 4096: 16.16.16,
-      8192: 32.16.16}, element e = k + r N/R of a radix-R stage at word k R + r.
+      8192: 2 (FP32 registers, put_r2) then 16.16.16 from LS = 2}, element
+      e = k + r N/R of a radix-R stage at word k R + r.
+  The FP32 azimuth kernel runs a radix-G first pass in registers (G = the
+      thread's output groups) and then Stockham passes from LS = G (fft_rest).
 """
 import numpy as np
 import pytest
@@ -131,7 +134,8 @@ def pp(p):
     return p + ((p >> 5) << 2) + ((p >> 9) << 2)
 
 
-MPLAN = {8: [16, 16], 9: [32, 16], 10: [32, 32], 12: [16, 16, 16], 13: [32, 16, 16]}
+MPLAN = {8: [16, 16], 9: [32, 16], 10: [32, 32], 12: [16, 16, 16], 13: [16, 16, 16]}
+LS0 = {13: 2}                 # radices the caller runs in registers before the put (put_r2)
 
 
 def _pos0(L, e):
@@ -149,7 +153,7 @@ def test_mfma_stage_epilogue(L):
     Rs, n = MPLAN[L], 1 << L
     for S in range(len(Rs) - 1):
         R, R2 = Rs[S], Rs[S + 1]
-        ls = int(np.prod(Rs[:S], dtype=np.int64)) if S else 1
+        ls = LS0.get(L, 1) * (int(np.prod(Rs[:S], dtype=np.int64)) if S else 1)
         nv, nv2, ls2 = n // R, n // R2, ls * R
         assert nv2 % ls2 == 0
         kb, C = nv2 // ls2, n // (ls2 * R2)
@@ -179,10 +183,11 @@ def _nw_for(L, ncol, bw):
     return min(bw, min(ncol * ((1 << L) // R) // 16 for R in MPLAN[L]))
 
 
-@pytest.mark.parametrize("L", sorted(MPLAN))
+@pytest.mark.parametrize("L", [L for L in sorted(MPLAN) if LS0.get(L, 1) == 1])
 def test_mfma_first_stage_stores(L):
     """pp(pos(e0 + d)) = pp(pos(e0)) + pp(pos(d)) for the azimuth kernel's
-    e0 = 4 t, d = 4 g NB + u and FFT-B's e0 = t, d = i NB."""
+    e0 = 4 t, d = 4 g NB + u and FFT-B's e0 = t, d = i NB (plans without a
+    register pass; put_r2 forms each position whole)."""
     n = 1 << L
     _, nt, _ = cfg(L, 8)
     nb = max(64, nt)
@@ -199,3 +204,79 @@ def test_mfma_first_stage_stores(L):
     for i in range(n // nbf):
         d = i * nbf
         assert np.array_equal(pp(_pos0(L, t + d)), pp(_pos0(L, t)) + pp(_pos0(L, d)))
+
+
+# ------------------------------------------------------------------ register first passes
+def stockham(x, radices, ls=1, pre_twiddled=False):
+    """Stockham passes as the kernels index them, in complex128: pass (R, LS)
+    reads elements k + r N/R (k < N/R), multiplies by W_N^((k % LS) r N/(LS R))
+    (unless the caller applied it: pre_twiddled, first pass only), runs the
+    R-point DFT and writes outputs (k / LS) LS R + k % LS + r LS."""
+    n = len(x)
+    buf = np.asarray(x, np.complex128)
+    for i, R in enumerate(radices):
+        nr = n // R
+        k = np.arange(nr)
+        j = k % ls
+        X = np.stack([buf[k + r * nr] for r in range(R)])
+        if ls > 1 and not (pre_twiddled and i == 0):
+            X = X * np.exp(-2j * np.pi * (j[None, :] * np.arange(R)[:, None] * (n // (ls * R))) / n)
+        Y = np.fft.fft(X, axis=0)
+        out = np.empty_like(buf)
+        o = (k // ls) * ls * R + j
+        for r in range(R):
+            out[o + r * ls] = Y[r]
+        buf, ls = out, ls * R
+    return buf
+
+
+def rest_radices(L, emax, ls):
+    """fft_rest: radix min(E, N / LS) from LS on, E = N / NT of Cfg<L, EMAX>."""
+    n, nt, _ = cfg(L, emax)
+    e = n // nt
+    out = []
+    while ls < n:
+        out.append(min(e, n // ls))
+        ls *= out[-1]
+    return out
+
+
+@pytest.mark.parametrize("L", [9, 10, 11, 12, 13, 14])
+def test_f32_register_first_pass(L):
+    """The azimuth kernel's FP32 FFT: a thread's G output groups hold points
+    k + g N/G (k = 4 t + u), so a radix-G pass with LS = 1 runs in registers and
+    stores outputs G k + r; fft_rest then continues from LS = G.  The whole is
+    the DFT."""
+    n = 1 << L
+    emax = 16 if L >= 14 else 8                      # az_emax
+    _, nt, _ = cfg(L, emax)
+    nb = max(64, nt)
+    G = (n + 4 * nb - 1) // (4 * nb)
+    assert G >= 2 and 4 * G * nb == n
+    x = np.random.default_rng(L).standard_normal(n) + 1j * np.random.default_rng(L + 1).standard_normal(n)
+    y = stockham(x, [G] + rest_radices(L, 16, G))
+    assert np.allclose(y, np.fft.fft(x), rtol=0, atol=1e-9 * np.abs(x).sum())
+
+
+def test_mfma_put_r2_twiddle():
+    """put_r2 at 8192 points: the odd output o of the radix-2 pass gets
+    W_32^r, r = o / (N/16), which is the first matrix stage's input twiddle
+    W_N^((k % 2) r step) of element o (k = o % (N/16), step = N/32), and the
+    even outputs get none; with (a +- b)/2 and 1/16 per matrix stage the
+    transform is DFT/N."""
+    L, n = 13, 1 << 13
+    nv = n // 16
+    e = np.arange(n)
+    jj, r = (e % nv) % 2, e // nv
+    assert np.array_equal(jj, e % 2)
+    put = np.where(e % 2 == 1, r, 0) * (n // 32) % n               # exponent put_r2 applies
+    stage = (jj * r * (n // 32)) % n                                # the stage's own exponent
+    assert np.array_equal(put, stage)
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    a, b = x[: n // 2], x[n // 2:]
+    line = np.empty(n, np.complex128)
+    line[0::2] = (a + b) / 2
+    line[1::2] = (a - b) / 2 * np.exp(-2j * np.pi * (np.arange(1, n, 2) // nv) / 32)
+    y = stockham(line, MPLAN[L], ls=LS0[L], pre_twiddled=True) / 16 ** len(MPLAN[L])
+    assert np.allclose(y, np.fft.fft(x) / n, rtol=0, atol=1e-12 * np.abs(x).sum())
