@@ -149,7 +149,7 @@ __global__ __launch_bounds__(af_adj_nb<L>()) void k_af_az_adj(
     const double *__restrict__ hm, const double *__restrict__ hn,
     const int16_t *__restrict__ sinc, const float2 *__restrict__ rot,
     const float2 *__restrict__ B, const float2 *__restrict__ twg, float *__restrict__ partial) {
-    constexpr int EM = az_emax<L>();
+    constexpr int EM = L >= 14 ? 16 : 8;              // the line FFT's elements a thread
     using Cf = pfa32::Cfg<L, EM>;
     constexpr int M2 = Cf::N;
     constexpr int NB = af_adj_nb<L>();

@@ -477,8 +477,10 @@ __device__ __forceinline__ int xcd_map(int b, int count) {
 // Elements per thread of the azimuth kernel's FFT and resample: 8, so an
 // 8192-point line runs on 1024 threads (16 waves keep the CU busy while one
 // workgroup holds the LDS).
+// 16384 points: 16 a thread on 1024 threads; 4096 points: 4 a thread, so they
+// too run on 1024 threads (the LDS holds one workgroup a CU either way)
 template <int L>
-constexpr int az_emax() { return L >= 14 ? 16 : 8; }   // 16384 points: 16 a thread on 1024 threads
+constexpr int az_emax() { return L >= 14 ? 16 : (L == 12 ? 4 : 8); }
 // The line FFT runs radix-16 passes (half the threads active) -- fewer LDS round
 // trips than radix 8 on all 1024.
 constexpr int AZ_FFT_EMAX = 16;
