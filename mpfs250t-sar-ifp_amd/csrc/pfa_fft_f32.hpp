@@ -179,7 +179,17 @@ __device__ __forceinline__ void pass(float2 *buf, const Tw &tw, Src *src, Sink &
             const int j = k % LS;
             if (LS > 1) {
                 constexpr int step = N / (LS * R);
-                if constexpr (R >= 8) {
+                if constexpr (LS == 2 && R == 16) {
+                    // W^(j r N/32) = W_32^(j r), j = k % 2: exact constants for j = 1
+                    if (j) {
+#pragma unroll
+                        for (int r = 1; r < R; ++r) {
+                            const float c = kC32[r][0], sn = INV ? kC32[r][1] : -kC32[r][1];
+                            x[b][r] = make_float2(fmaf(x[b][r].x, c, -x[b][r].y * sn),
+                                                  fmaf(x[b][r].x, sn, x[b][r].y * c));
+                        }
+                    }
+                } else if constexpr (R >= 8) {
                     // W^(j r step) for r = 4 a + c from two lookups, w1 = W^(j step)
                     // and w4 = W^(4 j step) (< N / 2), and at most four products deep:
                     // w2 = w1 w1, w3 = w2 w1, W^(4a) = w4^a, W^r = W^(4a) w_c.

@@ -396,16 +396,19 @@ __global__ __launch_bounds__(RT, 2) void k_range_f32(
         for (int g = 0; g < 2; ++g) {
             int d[4], ph[4];
             const int J0 = (code[g * 4] >> 8) - 1;
-            bool fast = SINC && p < m;
+            // every J in [max(15, J0), min(n - 17, J0 + KD)]: tested on the group's
+            // smallest and largest J; the last bin (the largest q) inside the tile
+            int jmin = J0, jmax = J0;
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int J = (code[g * 4 + u] >> 8) - 1;
                 d[u] = J - J0;
                 ph[u] = code[g * 4 + u] & 255;
-                fast = fast && J >= 15 && J <= n - 17 && d[u] >= 0 && d[u] <= KD &&
-                       q0 + 4 * (gi + g * (RT / RP)) + u < qe;
+                jmin = min(jmin, J);
+                jmax = max(jmax, J);
             }
-            fast = fast && J0 - 15 >= lo && J0 - 15 + KW <= hi;
+            const bool fast = SINC && p < m && q0 + 4 * (gi + g * (RT / RP)) + 3 < qe && jmin >= max(15, J0) &&
+                              jmax <= min(n - 17, J0 + KD) && J0 - 15 >= lo && J0 - 15 + KW <= hi;
             if (fast) {
                 float2 w4[4];
                 sinc4p(sp, J0 - 15 - lo, tp, ph, d, w4);
@@ -710,13 +713,23 @@ __global__ __launch_bounds__(az_nb<L>()) void k_azimuth_f32(
                 int d[4], ph[4];
                 bool fast = SINC;
                 const int J0 = (code[g * 4] >> 8) - 1;
+                // every J in [max(15, J0), min(m - 17, J0 + KD)]: the FP32 form tests
+                // the group's smallest and largest J; the matrix-core forms, at the
+                // register limit, test each J (no two extra live registers)
+                int jmin = J0, jmax = J0;
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int J = (code[g * 4 + u] >> 8) - 1;
                     d[u] = J - J0;
                     ph[u] = code[g * 4 + u] & 255;
-                    fast = fast && J >= 15 && J <= m - 17 && d[u] >= 0 && d[u] <= KD;
+                    if constexpr (FK == FK_F32) {
+                        jmin = min(jmin, J);
+                        jmax = max(jmax, J);
+                    } else {
+                        fast = fast && J >= 15 && J <= m - 17 && d[u] >= 0 && d[u] <= KD;
+                    }
                 }
+                if constexpr (FK == FK_F32) fast = fast && jmin >= max(15, J0) && jmax <= min(m - 17, J0 + KD);
                 if (fast) {
                     float2 w4[4];
                     if constexpr (BIG) sinc4p<6, true>(reg, J0 - 15, tapg, ph, d, w4);
